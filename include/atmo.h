@@ -1,0 +1,197 @@
+/*
+ * atmo.h — C ABI of the MI355X atmospheric-scattering precompute.
+ *
+ * Drop-in boundary for Proland's atmo preprocess
+ * (src/atmo/sources/proland/preprocess/atmo/PreprocessAtmo.{h,cpp}).
+ * Every entry point takes plain pointers and sizes; device pointers are
+ * HIP device addresses on the engine's device, `stream` is a hipStream_t
+ * passed as void* (NULL = the null stream).
+ *
+ * Table layouts (identical to the reference's .raw files, little-endian FP32):
+ *   transmittance  float[TRANSMITTANCE_H][TRANSMITTANCE_W][3]      (RGB)
+ *   irradiance     float[SKY_H][SKY_W][3]                          (RGB)
+ *   inscatter      float[RES_R][RES_MU][RES_NU*RES_MU_S][4]        (RGBA, x = nu*RES_MU_S + mu_s)
+ * (PreprocessAtmo.cpp:355-405 writes exactly these, followed by a 20-byte
+ *  trailer {0xCAFEBABE, width, height*depth, depth, components}.)
+ *
+ * Pass-level entry points (atmo_pass_*) work on the engine's internal
+ * "texture" layout: one float4 per texel (RGB tables leave .w unused), rows
+ * as above.  They mirror one reference shader program each and exist so a
+ * caller (or a test) can drive a single pass on its own inputs.
+ */
+#ifndef ATMO_H_
+#define ATMO_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ATMO_API __attribute__((visibility("default")))
+
+/* Status codes.  The reference reports no errors at all
+ * (PreprocessAtmo.cpp:367 uses fopen/fwrite unchecked); every entry point
+ * here returns one of these and sets atmo_last_error(). */
+typedef enum atmo_status {
+    ATMO_OK = 0,
+    ATMO_ERR_INVALID_ARGUMENT = 1,
+    ATMO_ERR_DEVICE = 2,        /* HIP runtime error, or no usable gfx950 device */
+    ATMO_ERR_OUT_OF_MEMORY = 3,
+    ATMO_ERR_IO = 4,            /* .raw file could not be written/read/validated */
+    ATMO_ERR_NUMERIC = 5,       /* NaN/Inf or out-of-range table values */
+    ATMO_ERR_UNSUPPORTED = 6
+} atmo_status;
+
+typedef struct atmo_vec3f { float x, y, z; } atmo_vec3f;
+
+/* Byte-compatible with proland::AtmoParameters (PreprocessAtmo.h:44-67):
+ * 24 x 4 bytes = 96 bytes, members in the same order. */
+typedef struct atmo_params {
+    float Rg;
+    float Rt;
+    float RL;
+    int TRANSMITTANCE_W;
+    int TRANSMITTANCE_H;
+    int SKY_W;
+    int SKY_H;
+    int RES_R;
+    int RES_MU;
+    int RES_MU_S;
+    int RES_NU;
+    float AVERAGE_GROUND_REFLECTANCE;
+    float HR;
+    atmo_vec3f betaR;
+    float HM;
+    atmo_vec3f betaMSca;
+    atmo_vec3f betaMEx;
+    float mieG;
+} atmo_params;
+
+/* Extensions the reference hard-codes (common.glsl:94-97 sample counts,
+ * PreprocessAtmo.cpp:351 order count).  Defaults reproduce the reference. */
+typedef struct atmo_options {
+    int orders;                 /* highest scattering order, >= 1 (reference: 4) */
+    int transmittance_samples;  /* TRANSMITTANCE_INTEGRAL_SAMPLES (500) */
+    int inscatter_samples;      /* INSCATTER_INTEGRAL_SAMPLES (50) */
+    int irradiance_samples;     /* IRRADIANCE_INTEGRAL_SAMPLES (32) */
+    int spherical_samples;      /* INSCATTER_SPHERICAL_INTEGRAL_SAMPLES (16) */
+    int flags;                  /* ATMO_FLAG_* */
+    int reserved[10];
+} atmo_options;
+
+#define ATMO_FLAG_CHECK_NUMERICS 1  /* validate outputs (finite, T in [0,1], E,S >= 0) */
+#define ATMO_FLAG_NO_GRAPH       2  /* launch passes eagerly instead of replaying a hipGraph */
+
+/* Reference defaults (PreprocessAtmo.cpp:90-98; note betaMEx = 4.44e-3). */
+ATMO_API void atmo_params_default(atmo_params *p);
+ATMO_API void atmo_options_default(atmo_options *o);
+
+/* Last error message of the calling thread ("" when none). */
+ATMO_API const char *atmo_last_error(void);
+
+/* Library version string. */
+ATMO_API const char *atmo_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* Whole-precompute entry points                                           */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces proland::preprocessAtmo(const AtmoParameters&, const char*)
+ * (PreprocessAtmo.h:75, PreprocessAtmo.cpp:424-436): if <outdir>/inscatter.raw
+ * exists (and its trailer validates) return ATMO_OK at once; otherwise run the
+ * 4-order precompute on the default device and write transmittance.raw,
+ * irradiance.raw and inscatter.raw (each via a temporary file + rename).
+ * Returns instead of calling exit(0). */
+ATMO_API atmo_status atmo_preprocess(const atmo_params *params, const char *outdir);
+
+/* Same, with explicit options; skip_if_exists=0 always recomputes. */
+ATMO_API atmo_status atmo_preprocess_ex(const atmo_params *params, const atmo_options *opts,
+                                        const char *outdir, int skip_if_exists);
+
+/* One parameter set, host output buffers in the .raw layouts above
+ * (any of the three pointers may be NULL to skip that table's copy-back). */
+ATMO_API atmo_status atmo_precompute_host(const atmo_params *params, const atmo_options *opts,
+                                          float *transmittance, float *irradiance,
+                                          float *inscatter);
+
+/* Engine: owns the device work tables and launch descriptors.  One engine per
+ * thread; calls on one engine are serialised by the caller. */
+typedef struct atmo_engine atmo_engine;
+
+ATMO_API atmo_status atmo_engine_create(atmo_engine **out, int device);
+ATMO_API atmo_status atmo_engine_destroy(atmo_engine *e);
+
+/* Device output pointers of one parameter set (.raw layouts). */
+typedef struct atmo_tables {
+    float *transmittance;
+    float *irradiance;
+    float *inscatter;
+} atmo_tables;
+
+/* Batched precompute of nsets independent parameter sets (each with its own
+ * table resolutions) into caller-allocated device tables out[0..nsets).
+ * Work tables are allocated (and kept) by the engine.  Replaces running the
+ * reference's preprocess once per set. */
+ATMO_API atmo_status atmo_engine_precompute(atmo_engine *e, const atmo_params *sets, int nsets,
+                                            const atmo_options *opts, const atmo_tables *out,
+                                            void *stream);
+
+/* Bytes of device work memory the engine would hold for this batch. */
+ATMO_API atmo_status atmo_engine_workspace_size(const atmo_params *sets, int nsets,
+                                                const atmo_options *opts, size_t *bytes);
+
+/* ---------------------------------------------------------------------- */
+/* Pass-level entry points (one reference shader program each)             */
+/* All pointers: device float4 tables on the engine's device.              */
+/* ---------------------------------------------------------------------- */
+
+/* transmittance.glsl:78-98 -> T[TRANSMITTANCE_H][TRANSMITTANCE_W] */
+ATMO_API atmo_status atmo_pass_transmittance(atmo_engine *e, const atmo_params *p,
+                                             const atmo_options *o, float *T4, void *stream);
+/* irradiance1.glsl:78-82 -> dE[SKY_H][SKY_W] */
+ATMO_API atmo_status atmo_pass_irradiance1(atmo_engine *e, const atmo_params *p,
+                                           const atmo_options *o, const float *T4, float *dE4,
+                                           void *stream);
+/* inscatter1.glsl:104-150 -> dSR, dSM [RES_R][RES_MU][RES_NU*RES_MU_S];
+ * if S4 != NULL also copyInscatter1.glsl:104-109: S = (dSR.rgb, dSM.r). */
+ATMO_API atmo_status atmo_pass_inscatter1(atmo_engine *e, const atmo_params *p,
+                                          const atmo_options *o, const float *T4, float *dSR4,
+                                          float *dSM4, float *S4, void *stream);
+/* inscatterS.glsl:111-189 -> dJ (first != 0: order 2, Rayleigh/Mie separate) */
+ATMO_API atmo_status atmo_pass_inscatterS(atmo_engine *e, const atmo_params *p,
+                                          const atmo_options *o, const float *T4,
+                                          const float *dE4, const float *dSR4, const float *dSM4,
+                                          int first, float *dJ4, void *stream);
+/* irradianceN.glsl:85-114 -> dE; if E4 != NULL also copyIrradiance k=1 + ADD blend:
+ * E += dE (PreprocessAtmo.cpp:335-341). */
+ATMO_API atmo_status atmo_pass_irradianceN(atmo_engine *e, const atmo_params *p,
+                                           const atmo_options *o, const float *dSR4,
+                                           const float *dSM4, int first, float *dE4, float *E4,
+                                           void *stream);
+/* inscatterN.glsl:105-131 -> dS (written to dSR4); if S4 != NULL also
+ * copyInscatterN.glsl:105-110 + ADD blend: S.rgb += dS / phaseR(nu). */
+ATMO_API atmo_status atmo_pass_inscatterN(atmo_engine *e, const atmo_params *p,
+                                          const atmo_options *o, const float *T4,
+                                          const float *dJ4, float *dSR4, float *S4, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* .raw files (PreprocessAtmo.cpp:355-405)                                  */
+/* ---------------------------------------------------------------------- */
+
+/* Writes width*height*depth(or 1)*components floats + the 20-byte trailer
+ * {0xCAFEBABE, width, height(*depth), depth (0 for 2D), components}. */
+ATMO_API atmo_status atmo_write_raw(const char *path, const float *data, int width, int height,
+                                    int depth, int components);
+/* Reads the trailer of a .raw file (header[5]) and validates magic + size. */
+ATMO_API atmo_status atmo_read_raw_header(const char *path, int32_t header[5]);
+/* Reads the payload of a validated .raw file into data (capacity in floats). */
+ATMO_API atmo_status atmo_read_raw(const char *path, float *data, size_t capacity_floats,
+                                   int32_t header[5]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ATMO_H_ */
