@@ -1,0 +1,194 @@
+// atmo_device.h — device-side data layout and math of the MI355X atmo precompute.
+//
+// Layout in HBM (one "set" = one atmosphere parameter set):
+//   T4   float4[TH][TW]                 transmittance (rgb, w unused)
+//   dE4  float4[SH][SW]                 irradiance delta of the current order
+//   E4   float4[SH][SW]                 irradiance accumulator
+//   dSR4, dSM4, dJ4  float4[RR][RMU][RNU*RMUS]   per-order deltas
+//   S4   float4[RR][RMU][RNU*RMUS]      inscatter accumulator == the .raw layout
+//   outT, outE  float[.][.][3]          RGB tables in the .raw layouts
+// Every table is addressed through a per-set descriptor, so one launch covers
+// any number of sets of mixed resolutions.
+//
+// Precision: the reference computes everything in FP32 GLSL.  Here the
+// cancellation-prone geometry (r^2 - Rg^2, the ray/ground discriminant, ray
+// lengths, sun-visibility tests, texel (mu, muS, nu)) is FP64 and the bulk
+// (table filtering, phase functions, transcendentals, accumulation) is FP32.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace atmo {
+
+// common.glsl:99 — the reference's value of pi.
+constexpr double kPI = 3.141592657;
+
+struct LayerGeom {
+    double r;        // FP32 uniform r of the layer (PreprocessAtmo.cpp:255-271)
+    double dhdH[4];  // FP32 uniform dhdH
+};
+
+struct SetDesc {
+    // FP32 parameters promoted to double
+    double Rg, Rt, RL, HR, HM, mieG, alpha;
+    double H;            // sqrt(Rt^2 - Rg^2)
+    double Rg2, Rt2, RL2;
+    float bR[3], bMS[3], bME[3];
+    // FP32 constants
+    float f_Rg, f_dRt, f_inv_dRt;   // Rg, Rt - Rg, 1/(Rt - Rg)
+    float f_H, f_invH;
+    float f_tMu;                    // tan(1.5) / 1.15  (getTransmittanceUV)
+    float f_tMuS;                   // tan(1.26 * 1.1)  (texture4D uMuS)
+    float kR;                       // 3 / (16 pi)
+    float kM;                       // 1.5 / (4 pi) (1 - g^2) / (2 + g^2)
+    float g2p1, m2g;                // 1 + g^2, -2 g
+    int TW, TH, SW, SH, RR, RMU, RMUS, RNU;
+    int W4;                         // RNU * RMUS
+    int n4;                         // RR * RMU * W4
+    // tables (device pointers)
+    float4 *T4, *dE4, *E4, *dSR4, *dSM4, *dJ4, *S4;
+    float *outT, *outE;
+    const LayerGeom *layers;
+};
+
+// ---------------------------------------------------------------------------
+// GL LINEAR + CLAMP_TO_EDGE along one axis (GL 3.3 core 3.8.11) for a texel
+// coordinate x = u * N - 0.5.  Out-of-range coordinates clamp first, which
+// yields the same edge texel as GL.
+__device__ __forceinline__ void gl_axis(float x, int N, int &i0, int &i1, float &a) {
+    x = fminf(fmaxf(x, -1.0f), (float)N);
+    float f = floorf(x);
+    a = x - f;
+    int k = (int)f;
+    i0 = min(max(k, 0), N - 1);
+    i1 = min(max(k + 1, 0), N - 1);
+}
+
+__device__ __forceinline__ void gl_axis_d(double x, int N, int &i0, int &i1, float &a) {
+    x = fmin(fmax(x, -1.0), (double)N);
+    double f = floor(x);
+    a = (float)(x - f);
+    int k = (int)f;
+    i0 = min(max(k, 0), N - 1);
+    i1 = min(max(k + 1, 0), N - 1);
+}
+
+__device__ __forceinline__ float3 f3(float4 v) { return make_float3(v.x, v.y, v.z); }
+__device__ __forceinline__ float3 lerp3(float3 a, float3 b, float t) {
+    return make_float3(fmaf(t, b.x - a.x, a.x), fmaf(t, b.y - a.y, a.y), fmaf(t, b.z - a.z, a.z));
+}
+__device__ __forceinline__ float3 operator*(float3 a, float3 b) {
+    return make_float3(a.x * b.x, a.y * b.y, a.z * b.z);
+}
+__device__ __forceinline__ float3 operator*(float3 a, float s) {
+    return make_float3(a.x * s, a.y * s, a.z * s);
+}
+__device__ __forceinline__ float3 operator+(float3 a, float3 b) {
+    return make_float3(a.x + b.x, a.y + b.y, a.z + b.z);
+}
+
+// Bilinear fetch from a float4 [H][W] table at texel coordinates (x, y).
+__device__ __forceinline__ float3 bilinear(const float4 *__restrict__ tab, int W, int H, float x,
+                                          float y) {
+    int i0, i1, j0, j1;
+    float a, b;
+    gl_axis(x, W, i0, i1, a);
+    gl_axis(y, H, j0, j1, b);
+    float3 v00 = f3(tab[j0 * W + i0]), v01 = f3(tab[j0 * W + i1]);
+    float3 v10 = f3(tab[j1 * W + i0]), v11 = f3(tab[j1 * W + i1]);
+    return lerp3(lerp3(v00, v01, a), lerp3(v10, v11, a), b);
+}
+
+// common.glsl:114-124 + 223-226: T(r, mu) with the altitude h = r - Rg given
+// (computed without cancellation by the caller).
+__device__ __forceinline__ float3 T_lookup(const SetDesc &s, float h, float mu) {
+    float uR = sqrtf(fmaxf(h, 0.0f) * s.f_inv_dRt);
+    float uMu = atanf((mu + 0.15f) * s.f_tMu) * (1.0f / 1.5f);
+    return bilinear(s.T4, s.TW, s.TH, uMu * s.TW - 0.5f, uR * s.TH - 0.5f);
+}
+
+// Phase functions, common.glsl:290-297.
+__device__ __forceinline__ float phaseR(const SetDesc &s, float mu) {
+    return s.kR * fmaf(mu, mu, 1.0f);
+}
+__device__ __forceinline__ float phaseM(const SetDesc &s, float mu) {
+    float x = fmaf(s.m2g, mu, s.g2p1);
+    float y = rsqrtf(x);
+    return s.kM * fmaf(mu, mu, 1.0f) * (y * y * y);
+}
+
+// common.glsl:209-219 limit(r, mu), FP64.
+__device__ __forceinline__ double limit_d(const SetDesc &s, double r, double mu) {
+    double rr1 = r * r * (mu * mu - 1.0);
+    double dout = -r * mu + sqrt(rr1 + s.RL2);
+    double delta2 = rr1 + s.Rg2;
+    if (delta2 >= 0.0) {
+        double din = -r * mu - sqrt(delta2);
+        if (din >= 0.0) dout = fmin(dout, din);
+    }
+    return dout;
+}
+
+// common.glsl:175-201 getMuMuSNu for texel (x, y) of a layer, FP64.
+__device__ __forceinline__ void mu_mus_nu(const SetDesc &s, const LayerGeom &L, int x, int y,
+                                          double &mu, double &muS, double &nu) {
+    double r = L.r;
+    double half = s.RMU * 0.5;
+    if (y < half) {
+        double d = 1.0 - y / (half - 1.0);
+        d = fmin(fmax(L.dhdH[2], d * L.dhdH[3]), L.dhdH[3] * 0.999);
+        mu = (s.Rg2 - r * r - d * d) / (2.0 * r * d);
+        double q = s.Rg / r;
+        mu = fmin(mu, -sqrt(1.0 - q * q) - 0.001);
+    } else {
+        double d = (y - half) / (half - 1.0);
+        d = fmin(fmax(L.dhdH[0], d * L.dhdH[1]), L.dhdH[1] * 0.999);
+        mu = (s.Rt2 - r * r - d * d) / (2.0 * r * d);
+    }
+    int xm = x % s.RMUS;
+    double m = xm / (s.RMUS - 1.0);
+    muS = tan((2.0 * m - 1.0 + 0.26) * 1.1) / tan(1.26 * 1.1);
+    nu = -1.0 + (double)(x / s.RMUS) / (s.RNU - 1.0) * 2.0;
+}
+
+// muS of a texel column (depends on x only).
+__device__ __forceinline__ double texel_mus(const SetDesc &s, int xm) {
+    double m = xm / (s.RMUS - 1.0);
+    return tan((2.0 * m - 1.0 + 0.26) * 1.1) / tan(1.26 * 1.1);
+}
+
+// texture4D coordinates (common.glsl:149-173) as texel coordinates:
+// tz (r axis), ty (mu axis), txs (mu_s axis inside a nu slice).  FP64 inputs.
+// rho2 = r^2 - Rg^2, rmu = r mu, delta = rmu^2 - rho2 (ray/ground discriminant).
+__device__ __forceinline__ void tex4d_rmu(const SetDesc &s, double rho2, double rmu,
+                                          double delta, double &tz, double &ty) {
+    double rho = sqrt(fmax(rho2, 0.0));
+    double f, c;
+    if (rmu < 0.0 && delta > 0.0) {
+        // (rmu + sqrt(delta)) / rho == -rho / (sqrt(delta) - rmu), finite at rho = 0
+        f = -rho / (sqrt(delta) - rmu);
+        c = 0.5 - 0.5 / s.RMU;
+    } else {
+        f = (sqrt(fmax(delta + s.H * s.H, 0.0)) - rmu) / (rho + s.H);
+        c = 0.5 + 0.5 / s.RMU;
+    }
+    double uMu = c + f * (0.5 - 1.0 / s.RMU);
+    ty = uMu * s.RMU - 0.5;
+    double uR = 0.5 / s.RR + rho / s.H * (1.0 - 1.0 / s.RR);
+    tz = uR * s.RR - 0.5;
+}
+
+__device__ __forceinline__ double tex4d_umus(const SetDesc &s, double muS) {
+    return 0.5 / s.RMUS +
+           (atan(fmax(muS, -0.1975) * tan(1.26 * 1.1)) / 1.1 + (1.0 - 0.26)) * 0.5 *
+               (1.0 - 1.0 / s.RMUS);
+}
+
+// FP32 uMuS texel offset inside a slice: uMuS * RMUS - 0.5.
+__device__ __forceinline__ float tex4d_txs_f(const SetDesc &s, float muS) {
+    float a = atanf(fmaxf(muS, -0.1975f) * s.f_tMuS) * (1.0f / 1.1f) + 0.74f;
+    return a * 0.5f * (float)(s.RMUS - 1);
+}
+
+}  // namespace atmo

@@ -1,0 +1,836 @@
+// atmo_engine.cpp — host side of the MI355X atmo precompute: parameter
+// validation, per-set descriptors, batch plans (tile tables + work tables +
+// captured hipGraph), the C ABI of include/atmo.h and the .raw file format.
+//
+// Pass order and blending follow PreprocessAtmo.cpp:273-354 (13-step state
+// machine), re-expressed as one kernel launch per pass for a whole batch:
+//   T, dE1 (+E=0), dS1 (+S=dS1), then per order n = 2..orders:
+//   dJn, dEn (+E+=dEn), dSn (+S+=dSn/phaseR).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include "../../include/atmo.h"
+#include "atmo_kernels.h"
+
+using namespace atmo;
+
+namespace {
+
+thread_local std::string g_err;
+
+atmo_status fail(atmo_status st, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return st;
+}
+
+#define HIPCHK(expr)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(e_ == hipErrorOutOfMemory ? ATMO_ERR_OUT_OF_MEMORY : ATMO_ERR_DEVICE, \
+                        "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+enum Pass { P_T = 0, P_E1, P_S1, P_J, P_EN, P_SN, P_CHECK, P_COUNT };
+
+atmo_options default_options() {
+    atmo_options o;
+    memset(&o, 0, sizeof o);
+    o.orders = 4;
+    o.transmittance_samples = 500;
+    o.inscatter_samples = 50;
+    o.irradiance_samples = 32;
+    o.spherical_samples = 16;
+    return o;
+}
+
+atmo_status validate(const atmo_params &p, int idx) {
+    auto bad = [&](const char *what) {
+        return fail(ATMO_ERR_INVALID_ARGUMENT, "parameter set %d: %s", idx, what);
+    };
+    if (!(p.Rg > 0.f) || !(p.Rt > p.Rg) || !(p.RL >= p.Rt)) return bad("need 0 < Rg < Rt <= RL");
+    if (p.TRANSMITTANCE_W < 2 || p.TRANSMITTANCE_H < 2) return bad("transmittance table < 2x2");
+    if (p.SKY_W < 2 || p.SKY_H < 2) return bad("irradiance table < 2x2");
+    if (p.RES_R < 2 || p.RES_MU < 4 || (p.RES_MU & 1) || p.RES_MU_S < 2 || p.RES_NU < 2)
+        return bad("inscatter resolution: need RES_R>=2, even RES_MU>=4, RES_MU_S>=2, RES_NU>=2");
+    if ((long long)p.RES_R * p.RES_MU * p.RES_MU_S * p.RES_NU > (1LL << 30))
+        return bad("inscatter table larger than 2^30 texels");
+    if ((long long)p.TRANSMITTANCE_W * p.TRANSMITTANCE_H > (1LL << 26) ||
+        (long long)p.SKY_W * p.SKY_H > (1LL << 24))
+        return bad("transmittance/irradiance table too large");
+    if (!(p.HR > 0.f) || !(p.HM > 0.f)) return bad("scale heights must be > 0");
+    if (!(p.mieG > -1.f && p.mieG < 1.f)) return bad("mieG must be in (-1, 1)");
+    const float *v = &p.betaR.x;
+    for (int i = 0; i < 3; ++i)
+        if (!(v[i] >= 0.f) || !(p.betaMSca.x >= 0.f) || !(p.betaMEx.x >= 0.f)) return bad("negative beta");
+    if (!(p.betaMSca.y >= 0.f && p.betaMSca.z >= 0.f && p.betaMEx.y >= 0.f && p.betaMEx.z >= 0.f))
+        return bad("negative beta");
+    if (!(p.AVERAGE_GROUND_REFLECTANCE >= 0.f)) return bad("negative ground reflectance");
+    return ATMO_OK;
+}
+
+atmo_status validate_opts(const atmo_options &o) {
+    if (o.orders < 1 || o.orders > 64) return fail(ATMO_ERR_INVALID_ARGUMENT, "orders must be in [1, 64]");
+    if (o.transmittance_samples < 1 || o.transmittance_samples > 1000000 || o.inscatter_samples < 1 ||
+        o.inscatter_samples > 100000)
+        return fail(ATMO_ERR_INVALID_ARGUMENT, "integral sample counts out of range");
+    if (o.irradiance_samples < 2 || (o.irradiance_samples & 1) || o.irradiance_samples > 256)
+        return fail(ATMO_ERR_INVALID_ARGUMENT, "irradiance_samples must be even, in [2, 256]");
+    if (o.spherical_samples < 1 || o.spherical_samples > 128)
+        return fail(ATMO_ERR_INVALID_ARGUMENT, "spherical_samples must be in [1, 128]");
+    return ATMO_OK;
+}
+
+// PreprocessAtmo.cpp:255-271 setLayer: computed in double from the float
+// parameters (float*float products stay float as in C++), uploaded as FP32.
+LayerGeom layer_geom(const atmo_params &p, int l) {
+    double r = l / (p.RES_R - 1.0);
+    r = r * r;
+    float RgRg = p.Rg * p.Rg, RtRt = p.Rt * p.Rt;
+    float dRR = RtRt - RgRg;
+    r = std::sqrt((double)RgRg + r * (double)dRR) +
+        (l == 0 ? 0.01 : (l == p.RES_R - 1 ? -0.001 : 0.0));
+    double dmin = p.Rt - r;
+    double dmax = std::sqrt(r * r - (double)RgRg) + std::sqrt((double)dRR);
+    double dminp = r - p.Rg;
+    double dmaxp = std::sqrt(r * r - (double)RgRg);
+    LayerGeom g;
+    g.r = (double)(float)r;
+    g.dhdH[0] = (double)(float)dmin;
+    g.dhdH[1] = (double)(float)dmax;
+    g.dhdH[2] = (double)(float)dminp;
+    g.dhdH[3] = (double)(float)dmaxp;
+    return g;
+}
+
+void fill_desc(SetDesc &s, const atmo_params &p) {
+    memset(&s, 0, sizeof s);
+    s.Rg = p.Rg; s.Rt = p.Rt; s.RL = p.RL;
+    s.HR = p.HR; s.HM = p.HM; s.mieG = p.mieG; s.alpha = p.AVERAGE_GROUND_REFLECTANCE;
+    s.H = std::sqrt(s.Rt * s.Rt - s.Rg * s.Rg);
+    s.Rg2 = s.Rg * s.Rg; s.Rt2 = s.Rt * s.Rt; s.RL2 = s.RL * s.RL;
+    s.bR[0] = p.betaR.x; s.bR[1] = p.betaR.y; s.bR[2] = p.betaR.z;
+    s.bMS[0] = p.betaMSca.x; s.bMS[1] = p.betaMSca.y; s.bMS[2] = p.betaMSca.z;
+    s.bME[0] = p.betaMEx.x; s.bME[1] = p.betaMEx.y; s.bME[2] = p.betaMEx.z;
+    s.f_Rg = (float)s.Rg;
+    s.f_dRt = (float)(s.Rt - s.Rg);
+    s.f_inv_dRt = (float)(1.0 / (s.Rt - s.Rg));
+    s.f_H = (float)s.H;
+    s.f_invH = (float)(1.0 / s.H);
+    s.f_tMu = (float)(std::tan(1.5) / 1.15);
+    s.f_tMuS = (float)std::tan(1.26 * 1.1);
+    s.kR = (float)(3.0 / (16.0 * kPI));
+    double g = s.mieG;
+    s.kM = (float)(1.5 * 1.0 / (4.0 * kPI) * (1.0 - g * g) / (2.0 + g * g));
+    s.g2p1 = (float)(1.0 + g * g);
+    s.m2g = (float)(-2.0 * g);
+    s.TW = p.TRANSMITTANCE_W; s.TH = p.TRANSMITTANCE_H;
+    s.SW = p.SKY_W; s.SH = p.SKY_H;
+    s.RR = p.RES_R; s.RMU = p.RES_MU; s.RMUS = p.RES_MU_S; s.RNU = p.RES_NU;
+    s.W4 = s.RMUS * s.RNU;
+    s.n4 = s.RR * s.RMU * s.W4;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct SetSizes {
+    size_t nT, nE, n4;
+};
+SetSizes sizes(const atmo_params &p) {
+    return {(size_t)p.TRANSMITTANCE_W * p.TRANSMITTANCE_H, (size_t)p.SKY_W * p.SKY_H,
+            (size_t)p.RES_R * p.RES_MU * p.RES_MU_S * p.RES_NU};
+}
+// work tables per set: T4, dE4, E4, dSR4, dSM4, dJ4
+size_t work_bytes(const atmo_params &p, const atmo_options &o) {
+    SetSizes z = sizes(p);
+    size_t b = align256(z.nT * 16) + 2 * align256(z.nE * 16) + 2 * align256(z.n4 * 16);
+    if (o.orders >= 2) b += align256(z.n4 * 16);  // dJ
+    return b;
+}
+
+bool same_params(const atmo_params &a, const atmo_params &b) { return memcmp(&a, &b, sizeof a) == 0; }
+bool same_opts(const atmo_options &a, const atmo_options &b) {
+    return a.orders == b.orders && a.transmittance_samples == b.transmittance_samples &&
+           a.inscatter_samples == b.inscatter_samples && a.irradiance_samples == b.irradiance_samples &&
+           a.spherical_samples == b.spherical_samples;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= n) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 256));
+        if (e == hipSuccess) n = bytes;
+        return e;
+    }
+};
+
+struct Plan {
+    std::vector<atmo_params> params;
+    atmo_options opts;
+    std::vector<SetDesc> hsets;
+    int off[P_COUNT], cnt[P_COUNT];
+    DevBuf dsets, dlayers, dtiles, ddirs, dflags, arena;
+    size_t dirJT_off = 0, dirJW_off = 0, dirE_off = 0;
+    size_t ldsJ1 = 0, ldsJ = 0, ldsE1 = 0, ldsEN = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    ~Plan() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+}  // namespace
+
+struct atmo_engine {
+    int device = 0;
+    hipStream_t cap = nullptr;  // capture stream
+    std::unique_ptr<Plan> plan;
+    DevBuf pass_sets, pass_layers, pass_tiles, pass_dirs;
+};
+
+namespace {
+
+atmo_status check_device(int device) {
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(ATMO_ERR_DEVICE, "device %d not present (%d devices)", device, n);
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(ATMO_ERR_DEVICE, "device %d is %s; this library is built for gfx950 only", device,
+                    prop.gcnArchName);
+    return ATMO_OK;
+}
+
+// Direction tables of the two spherical integrals (inscatterS.glsl:109-110,
+// 127-145; irradianceN.glsl:83-101), computed in double, stored as float.
+void build_dirs(int NS, int NI, std::vector<DirJHost> &jt, std::vector<float> &jw,
+                std::vector<DirEHost> &de) {
+    const double dth = kPI / NS, dph = kPI / NS;
+    jt.resize(NS);
+    jw.resize((size_t)NS * 2 * NS * 2);
+    for (int it = 0; it < NS; ++it) {
+        double th = (it + 0.5) * dth;
+        jt[it] = {(float)std::cos(th), (float)std::sin(th), (float)(dth * dph * std::sin(th)), 0.f};
+        for (int ip = 0; ip < 2 * NS; ++ip) {
+            double ph = (ip + 0.5) * dph;
+            jw[((size_t)it * 2 * NS + ip) * 2 + 0] = (float)(std::cos(ph) * std::sin(th));
+            jw[((size_t)it * 2 * NS + ip) * 2 + 1] = (float)(std::sin(ph) * std::sin(th));
+        }
+    }
+    const double eth = kPI / NI, eph = kPI / NI;
+    const int NTH = NI / 2;
+    de.resize((size_t)2 * NI * NTH);
+    for (int ip = 0; ip < 2 * NI; ++ip) {
+        double ph = (ip + 0.5) * eph;
+        for (int it = 0; it < NTH; ++it) {
+            double th = (it + 0.5) * eth;
+            double dw = eth * eph * std::sin(th);
+            double wz = std::cos(th);
+            de[(size_t)ip * NTH + it] = {(float)(std::cos(ph) * std::sin(th)), (float)wz,
+                                         (float)(wz * dw), 0.f};
+        }
+    }
+}
+
+atmo_status upload_dirs(DevBuf &buf, const atmo_options &o, size_t &offJT, size_t &offJW,
+                        size_t &offE) {
+    std::vector<DirJHost> jt;
+    std::vector<float> jw;
+    std::vector<DirEHost> de;
+    build_dirs(o.spherical_samples, o.irradiance_samples, jt, jw, de);
+    offJT = 0;
+    offJW = align256(jt.size() * sizeof(DirJHost));
+    offE = offJW + align256(jw.size() * sizeof(float));
+    size_t total = offE + align256(de.size() * sizeof(DirEHost));
+    HIPCHK(buf.ensure(total));
+    std::vector<char> h(total, 0);
+    memcpy(h.data() + offJT, jt.data(), jt.size() * sizeof(DirJHost));
+    memcpy(h.data() + offJW, jw.data(), jw.size() * sizeof(float));
+    memcpy(h.data() + offE, de.data(), de.size() * sizeof(DirEHost));
+    HIPCHK(hipMemcpy(buf.p, h.data(), total, hipMemcpyHostToDevice));
+    return ATMO_OK;
+}
+
+// Tile tables of every pass for a batch.  The spherical-integral passes list
+// the most expensive tiles first so mixed-resolution batches drain evenly.
+void build_tiles(const std::vector<atmo_params> &ps, const atmo_options &o,
+                 std::vector<uint2> &tiles, int off[P_COUNT], int cnt[P_COUNT]) {
+    tiles.clear();
+    const int n = (int)ps.size();
+    for (int pass = 0; pass < P_COUNT; ++pass) {
+        off[pass] = (int)tiles.size();
+        std::vector<std::pair<double, uint2>> v;
+        for (int i = 0; i < n; ++i) {
+            const atmo_params &p = ps[i];
+            SetSizes z = sizes(p);
+            long long nt = 0;
+            double cost = 0;
+            switch (pass) {
+            case P_T: nt = (z.nT + 15) / 16; break;
+            case P_E1: nt = (z.nE + 255) / 256; break;
+            case P_S1:
+            case P_SN: nt = (z.n4 + 255) / 256; break;
+            case P_J: nt = (long long)p.RES_R * p.RES_MU_S; cost = (double)p.RES_MU * p.RES_NU; break;
+            case P_EN: nt = (long long)z.nE; cost = (double)p.RES_NU; break;
+            case P_CHECK: nt = (std::max(std::max(z.n4, z.nT), z.nE) + 255) / 256; break;
+            }
+            if (pass == P_J && o.orders < 2) nt = 0;
+            if (pass == P_EN && o.orders < 2) nt = 0;
+            if (pass == P_SN && o.orders < 2) nt = 0;
+            for (long long t = 0; t < nt; ++t) v.push_back({cost, make_uint2((unsigned)i, (unsigned)t)});
+        }
+        if (pass == P_J || pass == P_EN)
+            std::stable_sort(v.begin(), v.end(),
+                             [](const std::pair<double, uint2> &a, const std::pair<double, uint2> &b) {
+                                 return a.first > b.first;
+                             });
+        for (auto &e : v) tiles.push_back(e.second);
+        cnt[pass] = (int)tiles.size() - off[pass];
+    }
+}
+
+atmo_status launch_all(Plan &pl, hipStream_t st) {
+    const SetDesc *S = (const SetDesc *)pl.dsets.p;
+    const uint2 *T = (const uint2 *)pl.dtiles.p;
+    const atmo_options &o = pl.opts;
+    const char *dirs = (const char *)pl.ddirs.p;
+    HIPCHK(launch_transmittance(S, T + pl.off[P_T], pl.cnt[P_T], o.transmittance_samples, st));
+    HIPCHK(launch_irradiance1(S, T + pl.off[P_E1], pl.cnt[P_E1], st));
+    HIPCHK(launch_inscatter1(S, T + pl.off[P_S1], pl.cnt[P_S1], o.inscatter_samples, st));
+    for (int order = 2; order <= o.orders; ++order) {
+        const bool first = order == 2;
+        HIPCHK(launch_inscatterS(S, T + pl.off[P_J], pl.cnt[P_J], o.spherical_samples,
+                                 dirs + pl.dirJT_off, dirs + pl.dirJW_off, first,
+                                 first ? pl.ldsJ1 : pl.ldsJ, st));
+        HIPCHK(launch_irradianceN(S, T + pl.off[P_EN], pl.cnt[P_EN], o.irradiance_samples,
+                                  dirs + pl.dirE_off, first, first ? pl.ldsE1 : pl.ldsEN, st));
+        HIPCHK(launch_inscatterN(S, T + pl.off[P_SN], pl.cnt[P_SN], o.inscatter_samples, st));
+    }
+    return ATMO_OK;
+}
+
+atmo_status build_plan(atmo_engine *e, const atmo_params *sets, int nsets, const atmo_options &o,
+                       std::unique_ptr<Plan> &out) {
+    std::unique_ptr<Plan> pl(new Plan);
+    pl->params.assign(sets, sets + nsets);
+    pl->opts = o;
+    // work arena
+    size_t total = 0;
+    std::vector<size_t> base(nsets);
+    for (int i = 0; i < nsets; ++i) {
+        base[i] = total;
+        total += work_bytes(sets[i], o);
+    }
+    HIPCHK(pl->arena.ensure(total));
+    // layers
+    std::vector<LayerGeom> layers;
+    std::vector<size_t> loff(nsets);
+    for (int i = 0; i < nsets; ++i) {
+        loff[i] = layers.size();
+        for (int l = 0; l < sets[i].RES_R; ++l) layers.push_back(layer_geom(sets[i], l));
+    }
+    HIPCHK(pl->dlayers.ensure(layers.size() * sizeof(LayerGeom)));
+    HIPCHK(hipMemcpy(pl->dlayers.p, layers.data(), layers.size() * sizeof(LayerGeom),
+                     hipMemcpyHostToDevice));
+    // descriptors (output pointers are filled in per call)
+    pl->hsets.resize(nsets);
+    char *ar = (char *)pl->arena.p;
+    for (int i = 0; i < nsets; ++i) {
+        SetDesc &s = pl->hsets[i];
+        fill_desc(s, sets[i]);
+        SetSizes z = sizes(sets[i]);
+        char *b = ar + base[i];
+        s.T4 = (float4 *)b; b += align256(z.nT * 16);
+        s.dE4 = (float4 *)b; b += align256(z.nE * 16);
+        s.E4 = (float4 *)b; b += align256(z.nE * 16);
+        s.dSR4 = (float4 *)b; b += align256(z.n4 * 16);
+        s.dSM4 = (float4 *)b; b += align256(z.n4 * 16);
+        s.dJ4 = o.orders >= 2 ? (float4 *)b : nullptr;
+        s.layers = (const LayerGeom *)pl->dlayers.p + loff[i];
+    }
+    HIPCHK(pl->dsets.ensure(nsets * sizeof(SetDesc)));
+    // tiles
+    std::vector<uint2> tiles;
+    build_tiles(pl->params, o, tiles, pl->off, pl->cnt);
+    HIPCHK(pl->dtiles.ensure(tiles.size() * sizeof(uint2)));
+    HIPCHK(hipMemcpy(pl->dtiles.p, tiles.data(), tiles.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    HIPCHK(pl->dflags.ensure(nsets * sizeof(unsigned)));
+    atmo_status st = upload_dirs(pl->ddirs, o, pl->dirJT_off, pl->dirJW_off, pl->dirE_off);
+    if (st != ATMO_OK) return st;
+    int maxNU = 0, maxSW = 0;
+    for (int i = 0; i < nsets; ++i) {
+        maxNU = std::max(maxNU, sets[i].RES_NU);
+        maxSW = std::max(maxSW, sets[i].SKY_W);
+    }
+    pl->ldsJ1 = lds_inscatterS(o.spherical_samples, maxNU, maxSW, true);
+    pl->ldsJ = lds_inscatterS(o.spherical_samples, maxNU, maxSW, false);
+    pl->ldsE1 = lds_irradianceN(o.irradiance_samples, maxNU, true);
+    pl->ldsEN = lds_irradianceN(o.irradiance_samples, maxNU, false);
+    if (pl->ldsJ1 > 160 * 1024 || pl->ldsE1 > 160 * 1024)
+        return fail(ATMO_ERR_UNSUPPORTED, "spherical staging needs %zu bytes of LDS (> 160 KiB)",
+                    std::max(pl->ldsJ1, pl->ldsE1));
+    (void)e;
+    out = std::move(pl);
+    return ATMO_OK;
+}
+
+atmo_status check_out(const atmo_tables &t, int i) {
+    if (!t.transmittance || !t.irradiance || !t.inscatter)
+        return fail(ATMO_ERR_INVALID_ARGUMENT, "output tables of set %d: NULL pointer", i);
+    if (((uintptr_t)t.inscatter & 15) != 0)
+        return fail(ATMO_ERR_INVALID_ARGUMENT, "inscatter output of set %d is not 16-byte aligned", i);
+    if (((uintptr_t)t.transmittance & 3) || ((uintptr_t)t.irradiance & 3))
+        return fail(ATMO_ERR_INVALID_ARGUMENT, "outputs of set %d are not 4-byte aligned", i);
+    return ATMO_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+void atmo_params_default(atmo_params *p) {
+    if (!p) return;
+    memset(p, 0, sizeof *p);
+    p->Rg = 6360.0f; p->Rt = 6420.0f; p->RL = 6421.0f;
+    p->TRANSMITTANCE_W = 256; p->TRANSMITTANCE_H = 64;
+    p->SKY_W = 64; p->SKY_H = 16;
+    p->RES_R = 32; p->RES_MU = 128; p->RES_MU_S = 32; p->RES_NU = 8;
+    p->AVERAGE_GROUND_REFLECTANCE = 0.1f;
+    p->HR = 8.0f;
+    p->betaR = {5.8e-3f, 1.35e-2f, 3.31e-2f};
+    p->HM = 1.2f;
+    p->betaMSca = {4e-3f, 4e-3f, 4e-3f};
+    p->betaMEx = {4.44e-3f, 4.44e-3f, 4.44e-3f};
+    p->mieG = 0.8f;
+}
+
+void atmo_options_default(atmo_options *o) {
+    if (o) *o = default_options();
+}
+
+const char *atmo_last_error(void) { return g_err.c_str(); }
+
+const char *atmo_version(void) { return "proland-atmo-mi355x 0.1 (gfx950)"; }
+
+atmo_status atmo_engine_create(atmo_engine **out, int device) {
+    if (!out) return fail(ATMO_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    atmo_status st = check_device(device);
+    if (st != ATMO_OK) return st;
+    HIPCHK(hipSetDevice(device));
+    atmo_engine *e = new atmo_engine;
+    e->device = device;
+    hipError_t he = hipStreamCreateWithFlags(&e->cap, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+        delete e;
+        return fail(ATMO_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(he));
+    }
+    *out = e;
+    g_err.clear();
+    return ATMO_OK;
+}
+
+atmo_status atmo_engine_destroy(atmo_engine *e) {
+    if (!e) return ATMO_OK;
+    (void)hipSetDevice(e->device);
+    (void)hipDeviceSynchronize();
+    e->plan.reset();
+    if (e->cap) (void)hipStreamDestroy(e->cap);
+    delete e;
+    return ATMO_OK;
+}
+
+atmo_status atmo_engine_workspace_size(const atmo_params *sets, int nsets, const atmo_options *opts,
+                                       size_t *bytes) {
+    if (!sets || nsets <= 0 || !bytes) return fail(ATMO_ERR_INVALID_ARGUMENT, "bad arguments");
+    atmo_options o = opts ? *opts : default_options();
+    size_t t = 0;
+    for (int i = 0; i < nsets; ++i) t += work_bytes(sets[i], o);
+    *bytes = t;
+    return ATMO_OK;
+}
+
+atmo_status atmo_engine_precompute(atmo_engine *e, const atmo_params *sets, int nsets,
+                                   const atmo_options *opts, const atmo_tables *out, void *stream) {
+    if (!e || !sets || nsets <= 0 || !out) return fail(ATMO_ERR_INVALID_ARGUMENT, "bad arguments");
+    atmo_options o = opts ? *opts : default_options();
+    atmo_status st = validate_opts(o);
+    if (st != ATMO_OK) return st;
+    for (int i = 0; i < nsets; ++i) {
+        if ((st = validate(sets[i], i)) != ATMO_OK) return st;
+        if ((st = check_out(out[i], i)) != ATMO_OK) return st;
+    }
+    HIPCHK(hipSetDevice(e->device));
+    hipStream_t hs = (hipStream_t)stream;
+    bool reuse = e->plan && (int)e->plan->params.size() == nsets && same_opts(e->plan->opts, o);
+    if (reuse)
+        for (int i = 0; i < nsets && reuse; ++i) reuse = same_params(e->plan->params[i], sets[i]);
+    if (!reuse) {
+        HIPCHK(hipStreamSynchronize(hs));
+        e->plan.reset();
+        if ((st = build_plan(e, sets, nsets, o, e->plan)) != ATMO_OK) return st;
+    }
+    Plan &pl = *e->plan;
+    bool changed = !reuse;
+    for (int i = 0; i < nsets; ++i) {
+        SetDesc &s = pl.hsets[i];
+        if (s.S4 != (float4 *)out[i].inscatter || s.outT != out[i].transmittance ||
+            s.outE != out[i].irradiance) {
+            s.S4 = (float4 *)out[i].inscatter;
+            s.outT = out[i].transmittance;
+            s.outE = out[i].irradiance;
+            changed = true;
+        }
+    }
+    if (changed) {
+        HIPCHK(hipStreamSynchronize(hs));
+        HIPCHK(hipMemcpy(pl.dsets.p, pl.hsets.data(), nsets * sizeof(SetDesc), hipMemcpyHostToDevice));
+    }
+    if (o.flags & ATMO_FLAG_NO_GRAPH) {
+        if ((st = launch_all(pl, hs)) != ATMO_OK) return st;
+    } else {
+        if (!pl.gexec) {
+            HIPCHK(hipStreamBeginCapture(e->cap, hipStreamCaptureModeThreadLocal));
+            atmo_status cs = launch_all(pl, e->cap);
+            hipGraph_t g = nullptr;
+            hipError_t ce = hipStreamEndCapture(e->cap, &g);
+            if (cs != ATMO_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return cs;
+            }
+            HIPCHK(ce);
+            pl.graph = g;
+            HIPCHK(hipGraphInstantiate(&pl.gexec, g, nullptr, nullptr, 0));
+        }
+        HIPCHK(hipGraphLaunch(pl.gexec, hs));
+    }
+    if (o.flags & ATMO_FLAG_CHECK_NUMERICS) {
+        HIPCHK(hipMemsetAsync(pl.dflags.p, 0, nsets * sizeof(unsigned), hs));
+        HIPCHK(launch_check((const SetDesc *)pl.dsets.p, (const uint2 *)pl.dtiles.p + pl.off[P_CHECK],
+                            pl.cnt[P_CHECK], (unsigned *)pl.dflags.p, hs));
+        std::vector<unsigned> flags(nsets);
+        HIPCHK(hipMemcpyAsync(flags.data(), pl.dflags.p, nsets * sizeof(unsigned), hipMemcpyDeviceToHost, hs));
+        HIPCHK(hipStreamSynchronize(hs));
+        for (int i = 0; i < nsets; ++i)
+            if (flags[i])
+                return fail(ATMO_ERR_NUMERIC, "set %d: %u texels non-finite or out of range", i, flags[i]);
+    }
+    g_err.clear();
+    return ATMO_OK;
+}
+
+atmo_status atmo_precompute_host(const atmo_params *params, const atmo_options *opts, float *T,
+                                 float *E, float *S) {
+    if (!params) return fail(ATMO_ERR_INVALID_ARGUMENT, "params is NULL");
+    atmo_status st = validate(*params, 0);
+    if (st != ATMO_OK) return st;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    atmo_engine *e = nullptr;
+    if ((st = atmo_engine_create(&e, dev)) != ATMO_OK) return st;
+    std::unique_ptr<atmo_engine, atmo_status (*)(atmo_engine *)> guard(e, atmo_engine_destroy);
+    SetSizes z = sizes(*params);
+    DevBuf out;
+    size_t bT = align256(z.nT * 12), bE = align256(z.nE * 12), bS = z.n4 * 16;
+    HIPCHK(out.ensure(bT + bE + bS));
+    atmo_tables t;
+    t.transmittance = (float *)out.p;
+    t.irradiance = (float *)((char *)out.p + bT);
+    t.inscatter = (float *)((char *)out.p + bT + bE);
+    atmo_options o = opts ? *opts : default_options();
+    o.flags |= ATMO_FLAG_NO_GRAPH;
+    if ((st = atmo_engine_precompute(e, params, 1, &o, &t, nullptr)) != ATMO_OK) return st;
+    HIPCHK(hipDeviceSynchronize());
+    if (T) HIPCHK(hipMemcpy(T, t.transmittance, z.nT * 12, hipMemcpyDeviceToHost));
+    if (E) HIPCHK(hipMemcpy(E, t.irradiance, z.nE * 12, hipMemcpyDeviceToHost));
+    if (S) HIPCHK(hipMemcpy(S, t.inscatter, z.n4 * 16, hipMemcpyDeviceToHost));
+    return ATMO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Pass-level entry points: one set, caller's float4 tables, synchronous.
+// ---------------------------------------------------------------------------
+static atmo_status pass_setup(atmo_engine *e, const atmo_params *p, const atmo_options *o,
+                              SetDesc &s, atmo_options &oo) {
+    if (!e || !p) return fail(ATMO_ERR_INVALID_ARGUMENT, "bad arguments");
+    oo = o ? *o : default_options();
+    atmo_status st = validate_opts(oo);
+    if (st != ATMO_OK) return st;
+    if ((st = validate(*p, 0)) != ATMO_OK) return st;
+    HIPCHK(hipSetDevice(e->device));
+    fill_desc(s, *p);
+    std::vector<LayerGeom> layers;
+    for (int l = 0; l < p->RES_R; ++l) layers.push_back(layer_geom(*p, l));
+    HIPCHK(e->pass_layers.ensure(layers.size() * sizeof(LayerGeom)));
+    HIPCHK(hipMemcpy(e->pass_layers.p, layers.data(), layers.size() * sizeof(LayerGeom), hipMemcpyHostToDevice));
+    s.layers = (const LayerGeom *)e->pass_layers.p;
+    return ATMO_OK;
+}
+
+static atmo_status pass_run(atmo_engine *e, const atmo_params *p, const atmo_options &o,
+                            const SetDesc &s, int pass, bool first, void *stream) {
+    hipStream_t hs = (hipStream_t)stream;
+    HIPCHK(hipStreamSynchronize(hs));
+    HIPCHK(e->pass_sets.ensure(sizeof(SetDesc)));
+    HIPCHK(hipMemcpy(e->pass_sets.p, &s, sizeof(SetDesc), hipMemcpyHostToDevice));
+    std::vector<uint2> tiles;
+    int off[P_COUNT], cnt[P_COUNT];
+    std::vector<atmo_params> ps(1, *p);
+    atmo_options ot = o;
+    ot.orders = std::max(ot.orders, 2);  // every pass gets its tiles
+    build_tiles(ps, ot, tiles, off, cnt);
+    HIPCHK(e->pass_tiles.ensure(tiles.size() * sizeof(uint2)));
+    HIPCHK(hipMemcpy(e->pass_tiles.p, tiles.data(), tiles.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    size_t oJT, oJW, oE;
+    atmo_status st = upload_dirs(e->pass_dirs, o, oJT, oJW, oE);
+    if (st != ATMO_OK) return st;
+    const SetDesc *S = (const SetDesc *)e->pass_sets.p;
+    const uint2 *T = (const uint2 *)e->pass_tiles.p;
+    const char *d = (const char *)e->pass_dirs.p;
+    switch (pass) {
+    case P_T: HIPCHK(launch_transmittance(S, T + off[P_T], cnt[P_T], o.transmittance_samples, hs)); break;
+    case P_E1: HIPCHK(launch_irradiance1(S, T + off[P_E1], cnt[P_E1], hs)); break;
+    case P_S1: HIPCHK(launch_inscatter1(S, T + off[P_S1], cnt[P_S1], o.inscatter_samples, hs)); break;
+    case P_J: {
+        size_t lds = lds_inscatterS(o.spherical_samples, p->RES_NU, p->SKY_W, first);
+        if (lds > 160 * 1024) return fail(ATMO_ERR_UNSUPPORTED, "LDS staging too large");
+        HIPCHK(launch_inscatterS(S, T + off[P_J], cnt[P_J], o.spherical_samples, d + oJT,
+                                 d + oJW, first, lds, hs));
+        break;
+    }
+    case P_EN: {
+        size_t lds = lds_irradianceN(o.irradiance_samples, p->RES_NU, first);
+        HIPCHK(launch_irradianceN(S, T + off[P_EN], cnt[P_EN], o.irradiance_samples, d + oE,
+                                  first, lds, hs));
+        break;
+    }
+    case P_SN: HIPCHK(launch_inscatterN(S, T + off[P_SN], cnt[P_SN], o.inscatter_samples, hs)); break;
+    }
+    HIPCHK(hipStreamSynchronize(hs));
+    g_err.clear();
+    return ATMO_OK;
+}
+
+static bool al16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
+atmo_status atmo_pass_transmittance(atmo_engine *e, const atmo_params *p, const atmo_options *o,
+                                    float *T4, void *stream) {
+    SetDesc s;
+    atmo_options oo;
+    atmo_status st = pass_setup(e, p, o, s, oo);
+    if (st != ATMO_OK) return st;
+    if (!T4 || !al16(T4)) return fail(ATMO_ERR_INVALID_ARGUMENT, "T4 must be a 16-byte aligned device pointer");
+    s.T4 = (float4 *)T4;
+    return pass_run(e, p, oo, s, P_T, false, stream);
+}
+
+atmo_status atmo_pass_irradiance1(atmo_engine *e, const atmo_params *p, const atmo_options *o,
+                                  const float *T4, float *dE4, void *stream) {
+    SetDesc s;
+    atmo_options oo;
+    atmo_status st = pass_setup(e, p, o, s, oo);
+    if (st != ATMO_OK) return st;
+    if (!T4 || !dE4 || !al16(T4) || !al16(dE4)) return fail(ATMO_ERR_INVALID_ARGUMENT, "bad table pointers");
+    s.T4 = (float4 *)T4;
+    s.dE4 = (float4 *)dE4;
+    return pass_run(e, p, oo, s, P_E1, false, stream);
+}
+
+atmo_status atmo_pass_inscatter1(atmo_engine *e, const atmo_params *p, const atmo_options *o,
+                                 const float *T4, float *dSR4, float *dSM4, float *S4, void *stream) {
+    SetDesc s;
+    atmo_options oo;
+    atmo_status st = pass_setup(e, p, o, s, oo);
+    if (st != ATMO_OK) return st;
+    if (!T4 || !dSR4 || !dSM4 || !al16(T4) || !al16(dSR4) || !al16(dSM4) || (S4 && !al16(S4)))
+        return fail(ATMO_ERR_INVALID_ARGUMENT, "bad table pointers");
+    s.T4 = (float4 *)T4;
+    s.dSR4 = (float4 *)dSR4;
+    s.dSM4 = (float4 *)dSM4;
+    s.S4 = (float4 *)S4;
+    return pass_run(e, p, oo, s, P_S1, false, stream);
+}
+
+atmo_status atmo_pass_inscatterS(atmo_engine *e, const atmo_params *p, const atmo_options *o,
+                                 const float *T4, const float *dE4, const float *dSR4,
+                                 const float *dSM4, int first, float *dJ4, void *stream) {
+    SetDesc s;
+    atmo_options oo;
+    atmo_status st = pass_setup(e, p, o, s, oo);
+    if (st != ATMO_OK) return st;
+    if (!T4 || !dE4 || !dSR4 || !dJ4 || (first && !dSM4) || !al16(T4) || !al16(dE4) || !al16(dSR4) ||
+        !al16(dJ4) || (dSM4 && !al16(dSM4)))
+        return fail(ATMO_ERR_INVALID_ARGUMENT, "bad table pointers");
+    s.T4 = (float4 *)T4;
+    s.dE4 = (float4 *)dE4;
+    s.dSR4 = (float4 *)dSR4;
+    s.dSM4 = (float4 *)(dSM4 ? dSM4 : dSR4);
+    s.dJ4 = dJ4 ? (float4 *)dJ4 : nullptr;
+    return pass_run(e, p, oo, s, P_J, first != 0, stream);
+}
+
+atmo_status atmo_pass_irradianceN(atmo_engine *e, const atmo_params *p, const atmo_options *o,
+                                  const float *dSR4, const float *dSM4, int first, float *dE4,
+                                  float *E4, void *stream) {
+    SetDesc s;
+    atmo_options oo;
+    atmo_status st = pass_setup(e, p, o, s, oo);
+    if (st != ATMO_OK) return st;
+    if (!dSR4 || !dE4 || (first && !dSM4) || !al16(dSR4) || !al16(dE4) || (dSM4 && !al16(dSM4)) ||
+        (E4 && !al16(E4)))
+        return fail(ATMO_ERR_INVALID_ARGUMENT, "bad table pointers");
+    s.dSR4 = (float4 *)dSR4;
+    s.dSM4 = (float4 *)(dSM4 ? dSM4 : dSR4);
+    s.dE4 = (float4 *)dE4;
+    s.E4 = (float4 *)E4;
+    return pass_run(e, p, oo, s, P_EN, first != 0, stream);
+}
+
+atmo_status atmo_pass_inscatterN(atmo_engine *e, const atmo_params *p, const atmo_options *o,
+                                 const float *T4, const float *dJ4, float *dSR4, float *S4,
+                                 void *stream) {
+    SetDesc s;
+    atmo_options oo;
+    atmo_status st = pass_setup(e, p, o, s, oo);
+    if (st != ATMO_OK) return st;
+    if (!T4 || !dJ4 || !dSR4 || !al16(T4) || !al16(dJ4) || !al16(dSR4) || (S4 && !al16(S4)))
+        return fail(ATMO_ERR_INVALID_ARGUMENT, "bad table pointers");
+    s.T4 = (float4 *)T4;
+    s.dJ4 = (float4 *)dJ4;
+    s.dSR4 = (float4 *)dSR4;
+    s.S4 = (float4 *)S4;
+    return pass_run(e, p, oo, s, P_SN, false, stream);
+}
+
+// ---------------------------------------------------------------------------
+// .raw files (PreprocessAtmo.cpp:355-405): FP32 payload + 20-byte trailer.
+// ---------------------------------------------------------------------------
+atmo_status atmo_write_raw(const char *path, const float *data, int width, int height, int depth,
+                           int components) {
+    if (!path || !data || width <= 0 || height <= 0 || depth < 0 || components <= 0)
+        return fail(ATMO_ERR_INVALID_ARGUMENT, "bad arguments");
+    size_t n = (size_t)width * height * (depth > 0 ? depth : 1) * components;
+    std::string tmp = std::string(path) + ".tmp";
+    FILE *f = fopen(tmp.c_str(), "wb");
+    if (!f) return fail(ATMO_ERR_IO, "cannot open %s: %s", tmp.c_str(), strerror(errno));
+    int32_t trailer[5] = {(int32_t)0xCAFEBABE, width, height * (depth > 0 ? depth : 1), depth,
+                          components};
+    bool ok = fwrite(data, sizeof(float), n, f) == n && fwrite(trailer, sizeof trailer, 1, f) == 1;
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) {
+        unlink(tmp.c_str());
+        return fail(ATMO_ERR_IO, "write failed: %s", tmp.c_str());
+    }
+    if (rename(tmp.c_str(), path) != 0) {
+        unlink(tmp.c_str());
+        return fail(ATMO_ERR_IO, "rename to %s failed: %s", path, strerror(errno));
+    }
+    return ATMO_OK;
+}
+
+atmo_status atmo_read_raw_header(const char *path, int32_t header[5]) {
+    if (!path || !header) return fail(ATMO_ERR_INVALID_ARGUMENT, "bad arguments");
+    FILE *f = fopen(path, "rb");
+    if (!f) return fail(ATMO_ERR_IO, "cannot open %s", path);
+    if (fseek(f, 0, SEEK_END) != 0) {
+        fclose(f);
+        return fail(ATMO_ERR_IO, "seek failed: %s", path);
+    }
+    long sz = ftell(f);
+    if (sz < 20) {
+        fclose(f);
+        return fail(ATMO_ERR_IO, "%s: too short for a trailer", path);
+    }
+    fseek(f, sz - 20, SEEK_SET);
+    size_t got = fread(header, sizeof(int32_t), 5, f);
+    fclose(f);
+    if (got != 5) return fail(ATMO_ERR_IO, "%s: cannot read trailer", path);
+    if ((uint32_t)header[0] != 0xCAFEBABEu) return fail(ATMO_ERR_IO, "%s: bad magic", path);
+    if (header[1] <= 0 || header[2] <= 0 || header[3] < 0 || header[4] <= 0)
+        return fail(ATMO_ERR_IO, "%s: bad trailer", path);
+    long long n = (long long)header[1] * header[2] * header[4];
+    if (header[3] > 0 && header[2] % header[3] != 0) return fail(ATMO_ERR_IO, "%s: bad depth", path);
+    if ((long long)sz != n * 4 + 20) return fail(ATMO_ERR_IO, "%s: size %ld does not match trailer", path, sz);
+    return ATMO_OK;
+}
+
+atmo_status atmo_read_raw(const char *path, float *data, size_t capacity, int32_t header[5]) {
+    atmo_status st = atmo_read_raw_header(path, header);
+    if (st != ATMO_OK) return st;
+    size_t n = (size_t)header[1] * header[2] * header[4];
+    if (!data || capacity < n) return fail(ATMO_ERR_INVALID_ARGUMENT, "buffer too small (%zu < %zu)", capacity, n);
+    FILE *f = fopen(path, "rb");
+    if (!f) return fail(ATMO_ERR_IO, "cannot open %s", path);
+    size_t got = fread(data, sizeof(float), n, f);
+    fclose(f);
+    if (got != n) return fail(ATMO_ERR_IO, "%s: short read", path);
+    return ATMO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// preprocessAtmo (PreprocessAtmo.cpp:424-436)
+// ---------------------------------------------------------------------------
+atmo_status atmo_preprocess_ex(const atmo_params *params, const atmo_options *opts, const char *outdir,
+                               int skip_if_exists) {
+    if (!params || !outdir) return fail(ATMO_ERR_INVALID_ARGUMENT, "bad arguments");
+    std::string dir(outdir);
+    std::string pS = dir + "/inscatter.raw", pT = dir + "/transmittance.raw", pE = dir + "/irradiance.raw";
+    if (skip_if_exists) {
+        int32_t h[5];
+        if (atmo_read_raw_header(pS.c_str(), h) == ATMO_OK) {
+            g_err.clear();
+            return ATMO_OK;  // cache hit, as the reference (inscatter.raw is written last)
+        }
+    }
+    atmo_status st = validate(*params, 0);
+    if (st != ATMO_OK) return st;
+    SetSizes z = sizes(*params);
+    std::vector<float> T(z.nT * 3), E(z.nE * 3), S(z.n4 * 4);
+    atmo_options o = opts ? *opts : default_options();
+    o.flags |= ATMO_FLAG_CHECK_NUMERICS;
+    if ((st = atmo_precompute_host(params, &o, T.data(), E.data(), S.data())) != ATMO_OK) return st;
+    const atmo_params &p = *params;
+    if ((st = atmo_write_raw(pT.c_str(), T.data(), p.TRANSMITTANCE_W, p.TRANSMITTANCE_H, 0, 3)) != ATMO_OK)
+        return st;
+    if ((st = atmo_write_raw(pE.c_str(), E.data(), p.SKY_W, p.SKY_H, 0, 3)) != ATMO_OK) return st;
+    if ((st = atmo_write_raw(pS.c_str(), S.data(), p.RES_MU_S * p.RES_NU, p.RES_MU, p.RES_R, 4)) != ATMO_OK)
+        return st;
+    g_err.clear();
+    return ATMO_OK;
+}
+
+atmo_status atmo_preprocess(const atmo_params *params, const char *outdir) {
+    return atmo_preprocess_ex(params, nullptr, outdir, 1);
+}
+
+}  // extern "C"

@@ -1,0 +1,37 @@
+// atmo_kernels.h — host-callable launchers of the gfx950 pass kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "atmo_device.h"
+
+namespace atmo {
+
+// Per-theta constants of the inscatterS direction grid (NS thetas).
+struct DirJHost {
+    float ct, st, dw, pad;
+};
+// Per-direction constants of the irradianceN hemisphere grid, index ip*(NI/2)+it.
+struct DirEHost {
+    float wx, wz, wzdw, pad;
+};
+
+size_t lds_inscatterS(int NS, int RNU, int SW, bool first);
+size_t lds_irradianceN(int NI, int RNU, bool first);
+
+hipError_t launch_transmittance(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
+                                hipStream_t st);
+hipError_t launch_irradiance1(const SetDesc *sets, const uint2 *tiles, int ntiles, hipStream_t st);
+hipError_t launch_inscatter1(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
+                             hipStream_t st);
+hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS,
+                             const void *dirT, const void *dirW, bool first, size_t lds,
+                             hipStream_t st);
+hipError_t launch_irradianceN(const SetDesc *sets, const uint2 *tiles, int ntiles, int NI,
+                              const void *dirs, bool first, size_t lds, hipStream_t st);
+hipError_t launch_inscatterN(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
+                             hipStream_t st);
+hipError_t launch_check(const SetDesc *sets, const uint2 *tiles, int ntiles, unsigned int *flags,
+                        hipStream_t st);
+
+}  // namespace atmo

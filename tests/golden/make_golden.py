@@ -1,0 +1,71 @@
+"""Writes the golden fixtures of the parity tests from the FP64 oracle.
+
+    python tests/golden/make_golden.py
+
+The reference ships no atmo tables or test vectors (the .raw outputs are
+excluded from its distribution, src/atmo/install.sh:17), so these vectors
+come from the pinned FP64 oracle (oracle/, checked in tests/test_oracle_pins.py):
+(S and T stored as float32: 6e-8 relative rounding, far below the 1e-3 bar)
+  tiny_earth.npz      Earth, T 64x16, E 16x8, inscatter 8x32x8x4, orders 4: T, E, S
+  tiny_seeds.npz      3 sets of the batch distribution (seed 0x5EED0003 stream),
+                      T 32x8, E 8x4, inscatter 6x16x6x4, orders 4
+  earth_ref.npz       the reference configuration (Earth, T 256x64, E 64x16,
+                      inscatter 32x128x32x8, 500/50/32/16 samples, orders 4):
+                      T and E in full, S at 4096 seeded texels
+"""
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.join(REPO, "proland-4.0_amd"))
+
+from oracle import oracle as O  # noqa: E402
+from prolandatmo.params import SEED_BATCH, draw_sets, earth, options, with_res  # noqa: E402
+
+
+def pvec(p):
+    return np.frombuffer(bytes(p), dtype=np.uint8).copy()
+
+
+def tiny_earth():
+    p = with_res(earth(), 8, 32, 8, 4)
+    p.TRANSMITTANCE_W, p.TRANSMITTANCE_H, p.SKY_W, p.SKY_H = 64, 16, 16, 8
+    T, E, S = O.precompute(p, options())
+    np.savez_compressed(os.path.join(HERE, "tiny_earth.npz"), params=pvec(p), T=T, E=E,
+                        S=S.astype(np.float32))
+
+
+def tiny_seeds():
+    sets, feats = draw_sets(SEED_BATCH, 3, res=(6, 16, 6, 4))
+    out = {}
+    for i, p in enumerate(sets):
+        p.TRANSMITTANCE_W, p.TRANSMITTANCE_H, p.SKY_W, p.SKY_H = 32, 8, 8, 4
+        T, E, S = O.precompute(p, options())
+        out["params%d" % i] = pvec(p)
+        out["T%d" % i], out["E%d" % i], out["S%d" % i] = T, E, S.astype(np.float32)
+    np.savez_compressed(os.path.join(HERE, "tiny_seeds.npz"), **out)
+
+
+def earth_ref():
+    p = earth()
+    t = time.time()
+    T, E, S = O.precompute(p, options())
+    print("earth reference config oracle: %.1f s on %d threads" % (time.time() - t, O.get_threads()))
+    rng = np.random.default_rng(0x5EED0001)
+    idx = rng.choice(S.shape[0] * S.shape[1] * S.shape[2], size=4096, replace=False)
+    idx.sort()
+    flat = S.reshape(-1, 4)
+    np.savez_compressed(os.path.join(HERE, "earth_ref.npz"), params=pvec(p), T=T.astype(np.float32),
+                        E=E, S_idx=idx.astype(np.int32), S_val=flat[idx])
+
+
+if __name__ == "__main__":
+    which = sys.argv[1:] or ["tiny_earth", "tiny_seeds", "earth_ref"]
+    for w in which:
+        globals()[w]()
+    print("ok")

@@ -1,0 +1,99 @@
+"""CPU tests of the C ABI library: it loads, exports every symbol include/*.h
+declares, validates its inputs, and handles .raw files (no GPU compute)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from prolandatmo import lib as L
+from prolandatmo.params import AtmoOptions, AtmoParams, earth, options
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    src = open(os.path.join(REPO, "include", header)).read()
+    return set(re.findall(r"ATMO_API\s+[^;(]*?\b(atmo_\w+)\s*\(", src, re.S))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared("atmo.h")
+    assert len(names) >= 20
+    lib = L.lib()
+    for n in names:
+        assert hasattr(lib, n), n
+    assert names == set(L.SIGNATURES), "binding and header disagree"
+    syms = os.popen("nm -D --defined-only %s" % L.LIB_PATH).read()
+    for mangled in ("_ZN7proland14preprocessAtmoERKNS_14AtmoParametersEPKc",
+                    "_ZN7proland14AtmoParametersC1Ev", "_ZN7proland20preprocessAtmoStatusEv"):
+        assert mangled in syms
+
+
+def test_params_layout_and_defaults():
+    assert ctypes.sizeof(AtmoParams) == 96
+    p = AtmoParams()
+    L.lib().atmo_params_default(p)
+    assert bytes(p) == bytes(earth())
+    o = AtmoOptions()
+    L.lib().atmo_options_default(o)
+    assert (o.orders, o.transmittance_samples, o.inscatter_samples, o.irradiance_samples,
+            o.spherical_samples) == (4, 500, 50, 32, 16)
+    assert L.lib().atmo_version().startswith(b"proland-atmo")
+
+
+def test_invalid_parameters_are_rejected_before_any_device_call():
+    bad = earth()
+    bad.Rt = bad.Rg - 1
+    with pytest.raises(L.AtmoError) as ei:
+        L.precompute_host(bad)
+    assert ei.value.status == 1 and "Rg" in str(ei.value)
+    odd = earth(RES_MU=127)
+    with pytest.raises(L.AtmoError):
+        L.precompute_host(odd)
+    st = L.lib().atmo_precompute_host(None, None, None, None, None)
+    assert st == 1
+
+
+def test_raw_round_trip_and_trailer(tmp_path):
+    rng = np.random.default_rng(0)
+    a = rng.random((4, 6, 10, 4)).astype(np.float32)  # depth 4, height 6, width 10, RGBA
+    path = str(tmp_path / "inscatter.raw")
+    L.write_raw(path, a, 10, 6, 4, 4)
+    raw = open(path, "rb").read()
+    assert len(raw) == a.size * 4 + 20
+    tr = np.frombuffer(raw[-20:], dtype=np.int32)
+    assert tr[0] == np.int32(np.uint32(0xCAFEBABE).view(np.int32))
+    assert list(tr[1:]) == [10, 24, 4, 4]  # width, height*depth, depth, components
+    assert np.array_equal(np.frombuffer(raw[:-20], dtype=np.float32), a.ravel())
+    data, h = L.read_raw(path)
+    assert np.array_equal(data, a.ravel()) and list(h[1:]) == [10, 24, 4, 4]
+    # 2D RGB table (transmittance/irradiance convention: depth 0)
+    t = rng.random((16, 64, 3)).astype(np.float32)
+    p2 = str(tmp_path / "t.raw")
+    L.write_raw(p2, t, 64, 16, 0, 3)
+    data, h = L.read_raw(p2)
+    assert list(h[1:]) == [64, 16, 0, 3] and np.array_equal(data, t.ravel())
+    # truncated file is rejected
+    open(p2, "r+b").truncate(100)
+    with pytest.raises(L.AtmoError):
+        L.read_raw(p2)
+
+
+def test_workspace_size_counts_work_tables():
+    p = earth()
+    n4 = 32 * 128 * 32 * 8
+    b = L.workspace_size(p, options())
+    assert b >= 3 * n4 * 16 and b < 4 * n4 * 16
+    b1 = L.workspace_size(p, options(orders=1))
+    assert b1 < b
+
+
+def test_engine_create_reports_missing_device():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    h = ctypes.c_void_p()
+    st = L.lib().atmo_engine_create(ctypes.byref(h), 0)
+    assert st == 2 and L.lib().atmo_last_error()

@@ -1,0 +1,260 @@
+"""Parity of the HIP path (libproland_atmo.so through its C ABI) with the FP64
+oracle, pass by pass and for the whole precompute.
+
+Bar (floating point): per table, max relative error <= 1e-3 against the FP64
+oracle, measured with prolandatmo.metrics.rel_err (relative error, floored at
+1e-4 of the table's largest magnitude).  Pass-level checks use tighter bars
+where the pass has no amplifying step.
+"""
+import os
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import oracle as O  # noqa: E402
+from prolandatmo import lib as L  # noqa: E402
+from prolandatmo.metrics import rel_err, summary  # noqa: E402
+from prolandatmo.params import (AtmoParams, SEED_BATCH, draw_sets, earth, options,  # noqa: E402
+                                with_res)
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TOL = 1e-3
+
+
+@pytest.fixture(scope="module")
+def eng():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    e = L.Engine(0)
+    yield e
+    e.close()
+
+
+def small(p=None, res=(8, 32, 8, 4), T=(64, 16), E=(16, 8)):
+    q = with_res(p or earth(), *res)
+    q.TRANSMITTANCE_W, q.TRANSMITTANCE_H = T
+    q.SKY_W, q.SKY_H = E
+    return q
+
+
+def f4(a):
+    return L.to_float4(a)
+
+
+def rgb(t):
+    return t[..., :3].double().cpu().numpy()
+
+
+def seeded(n, res, seed=SEED_BATCH):
+    sets, _ = draw_sets(seed, n, res=res)
+    for p in sets:
+        p.TRANSMITTANCE_W, p.TRANSMITTANCE_H, p.SKY_W, p.SKY_H = 64, 16, 16, 8
+    return sets
+
+
+CASES = [small(), small(res=(6, 16, 6, 4), T=(32, 8), E=(8, 4))] + seeded(2, (8, 32, 8, 4))
+
+
+def _assert(name, x, y, tol):
+    s = summary(x, y)
+    print("%-10s max %.3e mean %.3e (unfloored %.3e at %s)" % (name, s["max"], s["mean"],
+                                                               s["max_unfloored"], s["argmax"]),
+          flush=True)
+    assert s["max"] <= tol, (name, s)
+
+
+# ---------------------------------------------------------------- per pass
+@pytest.mark.parametrize("ci", range(len(CASES)))
+def test_pass_transmittance(eng, ci):
+    p = CASES[ci]
+    o = options()
+    T4 = torch.zeros((p.TRANSMITTANCE_H, p.TRANSMITTANCE_W, 4), device="cuda")
+    eng.pass_transmittance(p, o, T4)
+    _assert("T", rgb(T4), O.transmittance(p, o), 1e-5)
+
+
+@pytest.mark.parametrize("ci", range(len(CASES)))
+def test_pass_irradiance1_and_inscatter1(eng, ci):
+    p = CASES[ci]
+    o = options()
+    T = O.transmittance(p, o)
+    dE4 = torch.zeros((p.SKY_H, p.SKY_W, 4), device="cuda")
+    eng.pass_irradiance1(p, o, f4(T), dE4)
+    _assert("dE1", rgb(dE4), O.irradiance1(p, T, o), 1e-5)
+    shape = (p.RES_R, p.RES_MU, p.RES_NU * p.RES_MU_S, 4)
+    dSR4, dSM4, S4 = (torch.zeros(shape, device="cuda") for _ in range(3))
+    eng.pass_inscatter1(p, o, f4(T), dSR4, dSM4, S4)
+    oR, oM = O.inscatter1(p, T, o)
+    _assert("dSR1", rgb(dSR4), oR, 1e-4)
+    _assert("dSM1", rgb(dSM4), oM, 1e-4)
+    S = S4.double().cpu().numpy()
+    assert np.array_equal(S[..., :3], dSR4[..., :3].double().cpu().numpy())
+    assert np.array_equal(S[..., 3], dSM4[..., 0].double().cpu().numpy())
+
+
+@pytest.mark.parametrize("ci", range(len(CASES)))
+@pytest.mark.parametrize("first", [True, False])
+def test_pass_inscatterS_irradianceN_inscatterN(eng, ci, first):
+    p = CASES[ci]
+    o = options()
+    st = O.precompute_stages(p, options(orders=3))
+    T = st["T"]
+    if first:
+        dE, dSR, dSM = st["dE1"], st["dSR1"], st["dSM1"]
+    else:
+        dE, dSR, dSM = st["dE2"], st["dS2"], st["dSM1"]
+    dJ4 = torch.zeros((p.RES_R, p.RES_MU, p.RES_NU * p.RES_MU_S, 4), device="cuda")
+    eng.pass_inscatterS(p, o, f4(T), f4(dE), f4(dSR), f4(dSM), first, dJ4)
+    oJ = O.inscatterS(p, T, dE, dSR, dSM, first, o)
+    _assert("dJ", rgb(dJ4), oJ, 2e-4)
+    dEn4 = torch.zeros((p.SKY_H, p.SKY_W, 4), device="cuda")
+    E4 = f4(np.full((p.SKY_H, p.SKY_W, 3), 0.25))
+    eng.pass_irradianceN(p, o, f4(dSR), f4(dSM), first, dEn4, E4)
+    oE = O.irradianceN(p, dSR, dSM, first, o)
+    _assert("dEn", rgb(dEn4), oE, 2e-4)
+    assert np.allclose(rgb(E4), 0.25 + rgb(dEn4), rtol=1e-6, atol=0)
+    S0 = np.random.default_rng(ci).random((p.RES_R, p.RES_MU, p.RES_NU * p.RES_MU_S, 4))
+    S4 = torch.from_numpy(S0.astype(np.float32)).cuda()
+    dS4 = torch.zeros_like(S4)
+    eng.pass_inscatterN(p, o, f4(T), f4(oJ), dS4, S4)
+    oS = O.inscatterN(p, T, oJ, o)
+    _assert("dSn", rgb(dS4), oS, 2e-4)
+    oAcc = O.copy_inscatterN(p, oS, S0.astype(np.float32).astype(np.float64))
+    _assert("S+=", S4.double().cpu().numpy(), oAcc, 2e-4)
+
+
+# ---------------------------------------------------------------- whole precompute
+def _chain(eng, p, o=None):
+    (T, E, S), = eng.precompute(p, o or options())
+    torch.cuda.synchronize()
+    return T.double().cpu().numpy(), E.double().cpu().numpy(), S.double().cpu().numpy()
+
+
+@pytest.mark.parametrize("ci", range(len(CASES)))
+def test_chain_matches_oracle(eng, ci):
+    p = CASES[ci]
+    T, E, S = _chain(eng, p)
+    oT, oE, oS = O.precompute(p, options())
+    _assert("T", T, oT, TOL)
+    _assert("E", E, oE, TOL)
+    _assert("S", S, oS, TOL)
+
+
+def test_chain_orders_1_and_6(eng):
+    p = small()
+    for n in (1, 6):
+        o = options(orders=n)
+        T, E, S = _chain(eng, p, o)
+        oT, oE, oS = O.precompute(p, o)
+        _assert("S(o%d)" % n, S, oS, TOL)
+        if n == 1:
+            assert np.all(E == 0)
+        else:
+            _assert("E(o%d)" % n, E, oE, TOL)
+
+
+def test_chain_finer_sampling_options(eng):
+    p = small(res=(6, 16, 6, 4), T=(32, 8), E=(8, 4))
+    o = options(transmittance_samples=700, inscatter_samples=64, irradiance_samples=16,
+                spherical_samples=24, orders=3)
+    T, E, S = _chain(eng, p, o)
+    oT, oE, oS = O.precompute(p, o)
+    _assert("T", T, oT, TOL)
+    _assert("E", E, oE, TOL)
+    _assert("S", S, oS, TOL)
+
+
+def _params(v):
+    p = AtmoParams()
+    import ctypes
+    ctypes.memmove(ctypes.addressof(p), bytes(v), 96)
+    return p
+
+
+def test_golden_tiny_fixtures(eng):
+    g = np.load(os.path.join(GOLDEN, "tiny_earth.npz"))
+    p = _params(g["params"])
+    T, E, S = _chain(eng, p)
+    _assert("T", T, g["T"], TOL)
+    _assert("E", E, g["E"], TOL)
+    _assert("S", S, g["S"], TOL)
+    g = np.load(os.path.join(GOLDEN, "tiny_seeds.npz"))
+    for i in range(3):
+        p = _params(g["params%d" % i])
+        T, E, S = _chain(eng, p)
+        _assert("T%d" % i, T, g["T%d" % i], TOL)
+        _assert("E%d" % i, E, g["E%d" % i], TOL)
+        _assert("S%d" % i, S, g["S%d" % i], TOL)
+
+
+def test_reference_configuration_against_golden_samples(eng):
+    """The reference configuration (Earth, 32x128x32x8, 4 orders) against the
+    FP64 oracle: T and E in full, S at 4096 seeded texels."""
+    g = np.load(os.path.join(GOLDEN, "earth_ref.npz"))
+    p = _params(g["params"])
+    assert bytes(p) == bytes(earth())
+    T, E, S = _chain(eng, p)
+    _assert("T", T, g["T"], TOL)
+    _assert("E", E, g["E"], TOL)
+    flat = S.reshape(-1, 4)[g["S_idx"]]
+    # floor relative to the whole table's scale
+    scale = np.abs(S).max()
+    den = np.maximum(np.abs(g["S_val"]), 1e-4 * scale)
+    err = np.abs(flat - g["S_val"]) / den
+    print("S samples max %.3e mean %.3e" % (err.max(), err.mean()), flush=True)
+    assert err.max() <= TOL
+
+
+# ---------------------------------------------------------------- batching and graphs
+def test_mixed_resolution_batch_equals_single_runs(eng):
+    sets = seeded(3, None)
+    for p in sets:
+        p.RES_R, p.RES_MU, p.RES_MU_S, p.RES_NU = p.RES_R // 2, p.RES_MU // 2, p.RES_MU_S // 2, max(2, p.RES_NU // 2)
+    outs = eng.precompute(sets, options())
+    torch.cuda.synchronize()
+    for p, (T, E, S) in zip(sets, outs):
+        T1, E1, S1 = _chain(eng, p)
+        assert np.array_equal(T.double().cpu().numpy(), T1)
+        assert np.array_equal(E.double().cpu().numpy(), E1)
+        assert np.array_equal(S.double().cpu().numpy(), S1)
+
+
+def test_graph_replays_match_eager(eng):
+    p = small()
+    eager = _chain(eng, p, options(flags=2))
+    outs = L.alloc_outputs(p)
+    for rep in range(3):
+        for t in outs:
+            t.fill_(-7.0)
+        torch.cuda.synchronize()
+        eng.precompute(p, options(), outs=[outs])
+        torch.cuda.synchronize()
+        x = tuple(t.double().cpu().numpy() for t in outs)
+        torch.cuda.synchronize()
+        for a, b in zip(x, eager):
+            assert np.array_equal(a, b), rep
+
+
+def test_numeric_check_flag_passes_on_valid_tables(eng):
+    p = small()
+    eng.precompute(p, options(flags=1))
+
+
+def test_preprocess_writes_reference_files(eng, tmp_path):
+    p = small()
+    d = str(tmp_path)
+    L.preprocess(p, d)
+    T, E, S = _chain(eng, p)
+    for name, ref, shape in (("transmittance", T, [p.TRANSMITTANCE_W, p.TRANSMITTANCE_H, 0, 3]),
+                             ("irradiance", E, [p.SKY_W, p.SKY_H, 0, 3]),
+                             ("inscatter", S, [p.RES_MU_S * p.RES_NU, p.RES_MU * p.RES_R, p.RES_R, 4])):
+        data, h = L.read_raw(os.path.join(d, name + ".raw"))
+        assert list(h[1:]) == shape
+        assert np.array_equal(data, ref.astype(np.float32).ravel())
+    m = os.path.getmtime(os.path.join(d, "inscatter.raw"))
+    L.preprocess(p, d)  # cached: returns without recomputing
+    assert os.path.getmtime(os.path.join(d, "inscatter.raw")) == m
