@@ -1,0 +1,188 @@
+"""Benchmark of the MI355X atmo precompute.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-extra]
+
+One step = one full precompute (transmittance, irradiance and 4D inscatter
+tables, all scattering orders) of the high-resolution Earth workload: inscatter
+64x256x64x16 (RES_R x RES_MU x RES_MU_S x RES_NU), transmittance 512x128,
+irradiance 128x32, 32x64 spherical-integral directions, orders 1-8, 500/50/32
+integration samples.  Inputs are parameters only; every table lives in HBM.
+Prints one JSON line (rank 0).  Only one GPU is supported.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+for _p in (REPO, os.path.join(REPO, "proland-4.0_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (packed) peak, AMD public spec
+HBM_PEAK_GBS = 8000.0
+
+
+def workload_params():
+    from prolandatmo.params import earth, options
+    p = earth(TRANSMITTANCE_W=512, TRANSMITTANCE_H=128, SKY_W=128, SKY_H=32,
+              RES_R=64, RES_MU=256, RES_MU_S=64, RES_NU=16)
+    o = options(orders=8, spherical_samples=32)
+    return p, o
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_steps(eng, sets, o, outs, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        eng.precompute(sets, o, outs=outs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.precompute(sets, o, outs=outs)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def cpu_baseline(p, o, budget_threads=16):
+    """FP64 CPU oracle on a bounded sample of the same workload: the full-size
+    transmittance and irradiance passes plus the 4D passes at 8x32x16x8
+    (1/512 of the texels), extrapolated linearly in texel count."""
+    import numpy as np
+    from oracle import oracle as O
+    from prolandatmo.params import options, with_res
+    threads = min(budget_threads, os.cpu_count() or 1)
+    O.set_threads(threads)
+    q = with_res(p, 8, 32, 16, 8)
+    scale = (p.RES_R * p.RES_MU * p.RES_MU_S * p.RES_NU) / float(q.RES_R * q.RES_MU * q.RES_MU_S * q.RES_NU)
+    t = {}
+    c = time.perf_counter
+    a = c(); T = O.transmittance(q, o); t["T"] = c() - a
+    a = c(); dE = O.irradiance1(q, T, o); t["E1"] = c() - a
+    a = c(); dSR, dSM = O.inscatter1(q, T, o); t["S1"] = c() - a
+    t4 = t["S1"]
+    tE = t["T"] + t["E1"]
+    S = O.copy_inscatter1(q, dSR, dSM)
+    for n in range(2, o.orders + 1):
+        first = n == 2
+        a = c(); dJ = O.inscatterS(q, T, dE, dSR, dSM, first, o); t4 += c() - a
+        a = c(); dE = O.irradianceN(q, dSR, dSM, first, o); tE += c() - a
+        a = c(); dSR = O.inscatterN(q, T, dJ, o); t4 += c() - a
+        a = c(); S = O.copy_inscatterN(q, dSR, S); t4 += c() - a
+    est = tE + t4 * scale
+    return {"value": 1.0 / est, "unit": "precomputes/s", "cores": threads, "kind": "port",
+            "sample": ("FP64 oracle (oracle/, builder's CPU restatement; the reference GL preprocess "
+                       "cannot run here) on the same workload with the 4D table reduced to 8x32x16x8 "
+                       "(1/%d of the texels), full-size T/E passes; %.1f s measured, %.0f s "
+                       "extrapolated per precompute" % (int(scale), tE + t4, est))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also copy a 256 MiB device buffer (HBM counter calibration under rocprofv3)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the precompute shards by parameter set with no "
+              "communication, and multi-GPU runs are not part of this build (got --gpus %d)"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from prolandatmo import flops as F
+    from prolandatmo import lib as L
+    from prolandatmo.params import FLAG_TIME_PASSES, earth, options
+
+    torch.cuda.set_device(0)
+    eng = L.Engine(0)
+    p, o = workload_params()
+    outs = [L.alloc_outputs(p)]
+    log("workload: 64x256x64x16, orders 8, 32x64 directions")
+    sec = time_steps(eng, [p], o, outs, args.steps, args.warmup)
+    value = 1.0 / sec
+    log("%.2f ms per precompute" % (sec * 1e3))
+
+    # per-pass event timing of one eager run on the same stream
+    ot = options(orders=o.orders, spherical_samples=o.spherical_samples, flags=FLAG_TIME_PASSES)
+    eng.precompute([p], ot, outs=outs)
+    torch.cuda.synchronize()
+    times = eng.pass_times()
+    by = {}
+    for name, order, ms in times:
+        by.setdefault(name, []).append(ms)
+    pass_ms = {k: sum(v) for k, v in by.items()}
+    dom = max(pass_ms, key=pass_ms.get)
+    work = F.per_pass(p, o)
+    launches = len(by[dom])
+    avg_ms = pass_ms[dom] / launches
+    flop_per_launch = work[dom] / launches
+    achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12
+    roof = {"bound": "valu", "achieved": round(achieved, 2), "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+            "kernel": "k_inscatterS (%s pass)" % dom if dom == "inscatterS" else dom,
+            "avg_launch_ms": round(avg_ms, 4), "flop_per_launch": flop_per_launch,
+            "flop_convention": "reference GLSL formulation (prolandatmo/flops.py)"}
+    whole_tflops = F.total(p, o) / sec / 1e12
+
+    extra = {"pass_ms": {k: round(v, 3) for k, v in pass_ms.items()},
+             "algorithmic_tflops_whole_precompute": round(whole_tflops, 2)}
+    if not args.no_extra:
+        # the reference configuration (Earth, 32x128x32x8, 4 orders) and its
+        # accuracy at the committed FP64-oracle golden texels
+        pr = earth()
+        orr = options()
+        outs1 = [L.alloc_outputs(pr)]
+        sec1 = time_steps(eng, [pr], orr, outs1, 20, 2)
+        g = np.load(os.path.join(REPO, "tests", "golden", "earth_ref.npz"))
+        T, E, S = (t.double().cpu().numpy() for t in outs1[0])
+        from prolandatmo.metrics import summary
+        scale = np.abs(S).max()
+        den = np.maximum(np.abs(g["S_val"]), 1e-4 * scale)
+        errS = float((np.abs(S.reshape(-1, 4)[g["S_idx"]] - g["S_val"]) / den).max())
+        extra["reference_config"] = {
+            "workload": "Earth 32x128x32x8, T 256x64, E 64x16, 16x32 dirs, orders 4",
+            "precomputes_per_s": round(1.0 / sec1, 2), "ms_per_precompute": round(sec1 * 1e3, 3),
+            "max_rel_err_vs_fp64_oracle": {"T": summary(T, g["T"])["max"],
+                                           "E": summary(E, g["E"])["max"],
+                                           "S_4096_texels": errS}}
+        log("reference config: %.3f ms" % (sec1 * 1e3))
+        del outs1
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(p, o)
+    out = {
+        "metric": "precomputes/sec at <=1e-3 rel err vs FP64; surrogate vs plain speedup",
+        "value": round(value, 4), "unit": "precomputes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 (fp64 geometry)",
+        "data": "synthetic (atmosphere parameters only)",
+        "config": {"workload": "Earth precision-study precompute: inscatter 64x256x64x16, "
+                               "transmittance 512x128, irradiance 128x32, 32x64 spherical "
+                               "directions, orders 1-8, 500/50/32 samples",
+                   "parallelism": "single GPU"},
+        "roofline": roof, "cpu_baseline": cpu, "extra": extra,
+    }
+    print(json.dumps(out))
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copied %d bytes" % (src.numel() * 4))
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

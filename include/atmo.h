@@ -83,6 +83,8 @@ typedef struct atmo_options {
 
 #define ATMO_FLAG_CHECK_NUMERICS 1  /* validate outputs (finite, T in [0,1], E,S >= 0) */
 #define ATMO_FLAG_NO_GRAPH       2  /* launch passes eagerly instead of replaying a hipGraph */
+#define ATMO_FLAG_TIME_PASSES    4  /* eager launches, a hipEvent pair around each pass
+                                       (read with atmo_engine_pass_times) */
 
 /* Reference defaults (PreprocessAtmo.cpp:90-98; note betaMEx = 4.44e-3). */
 ATMO_API void atmo_params_default(atmo_params *p);
@@ -137,6 +139,18 @@ typedef struct atmo_tables {
 ATMO_API atmo_status atmo_engine_precompute(atmo_engine *e, const atmo_params *sets, int nsets,
                                             const atmo_options *opts, const atmo_tables *out,
                                             void *stream);
+
+/* Per-pass GPU times of the last atmo_engine_precompute run with
+ * ATMO_FLAG_TIME_PASSES (hipEvents on the caller's stream).  pass: 0 transmittance,
+ * 1 irradiance1, 2 inscatter1, 3 inscatterS, 4 irradianceN, 5 inscatterN;
+ * order: the scattering order the launch computes (1 for the first three). */
+typedef struct atmo_pass_time {
+    int pass;
+    int order;
+    float ms;
+} atmo_pass_time;
+ATMO_API atmo_status atmo_engine_pass_times(atmo_engine *e, atmo_pass_time *out, int capacity,
+                                            int *count);
 
 /* Bytes of device work memory the engine would hold for this batch. */
 ATMO_API atmo_status atmo_engine_workspace_size(const atmo_params *sets, int nsets,

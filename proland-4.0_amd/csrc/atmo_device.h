@@ -75,6 +75,16 @@ __device__ __forceinline__ void gl_axis_d(double x, int N, int &i0, int &i1, flo
 }
 
 __device__ __forceinline__ float3 f3(float4 v) { return make_float3(v.x, v.y, v.z); }
+
+// Table pointers reach the kernels through the set descriptors, so the
+// compiler sees generic (flat) pointers; this re-declares them as global
+// memory so loads use global_load with a scalar base.
+struct F4 {
+    float x, y, z, w;
+};
+typedef __attribute__((address_space(1))) const F4 gfloat4;
+__device__ __forceinline__ const gfloat4 *gptr(const float4 *p) { return (const gfloat4 *)p; }
+__device__ __forceinline__ float3 ld3(const gfloat4 *p) { return make_float3(p->x, p->y, p->z); }
 __device__ __forceinline__ float3 lerp3(float3 a, float3 b, float t) {
     return make_float3(fmaf(t, b.x - a.x, a.x), fmaf(t, b.y - a.y, a.y), fmaf(t, b.z - a.z, a.z));
 }
@@ -95,8 +105,9 @@ __device__ __forceinline__ float3 bilinear(const float4 *__restrict__ tab, int W
     float a, b;
     gl_axis(x, W, i0, i1, a);
     gl_axis(y, H, j0, j1, b);
-    float3 v00 = f3(tab[j0 * W + i0]), v01 = f3(tab[j0 * W + i1]);
-    float3 v10 = f3(tab[j1 * W + i0]), v11 = f3(tab[j1 * W + i1]);
+    const gfloat4 *t = gptr(tab);
+    float3 v00 = ld3(t + j0 * W + i0), v01 = ld3(t + j0 * W + i1);
+    float3 v10 = ld3(t + j1 * W + i0), v11 = ld3(t + j1 * W + i1);
     return lerp3(lerp3(v00, v01, a), lerp3(v10, v11, a), b);
 }
 

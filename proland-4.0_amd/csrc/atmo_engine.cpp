@@ -70,8 +70,10 @@ atmo_status validate(const atmo_params &p, int idx) {
     if (p.SKY_W < 2 || p.SKY_H < 2) return bad("irradiance table < 2x2");
     if (p.RES_R < 2 || p.RES_MU < 4 || (p.RES_MU & 1) || p.RES_MU_S < 2 || p.RES_NU < 2)
         return bad("inscatter resolution: need RES_R>=2, even RES_MU>=4, RES_MU_S>=2, RES_NU>=2");
-    if ((long long)p.RES_R * p.RES_MU * p.RES_MU_S * p.RES_NU > (1LL << 30))
-        return bad("inscatter table larger than 2^30 texels");
+    if ((long long)p.RES_R * p.RES_MU * p.RES_MU_S * p.RES_NU > (1LL << 27))
+        return bad("inscatter table larger than 2^27 texels (2 GiB per work table)");
+    if (p.RES_MU_S * p.RES_NU > 1024) return bad("RES_MU_S * RES_NU must be <= 1024");
+    if (p.RES_NU > 64) return bad("RES_NU must be <= 64");
     if ((long long)p.TRANSMITTANCE_W * p.TRANSMITTANCE_H > (1LL << 26) ||
         (long long)p.SKY_W * p.SKY_H > (1LL << 24))
         return bad("transmittance/irradiance table too large");
@@ -212,6 +214,7 @@ struct atmo_engine {
     hipStream_t cap = nullptr;  // capture stream
     std::unique_ptr<Plan> plan;
     DevBuf pass_sets, pass_layers, pass_tiles, pass_dirs;
+    std::vector<atmo_pass_time> times;
 };
 
 namespace {
@@ -295,8 +298,14 @@ void build_tiles(const std::vector<atmo_params> &ps, const atmo_options &o,
             switch (pass) {
             case P_T: nt = (z.nT + 15) / 16; break;
             case P_E1: nt = (z.nE + 255) / 256; break;
-            case P_S1:
-            case P_SN: nt = (z.n4 + 255) / 256; break;
+            case P_S1: nt = (z.n4 + 255) / 256; break;
+            case P_SN: {
+                // workgroups of up to 4 mu rows (k_inscatterN)
+                const int W4 = p.RES_MU_S * p.RES_NU;
+                const int rpw = std::max(1, std::min(4, 256 / W4));
+                nt = (long long)p.RES_R * ((p.RES_MU + rpw - 1) / rpw);
+                break;
+            }
             case P_J: nt = (long long)p.RES_R * p.RES_MU_S; cost = (double)p.RES_MU * p.RES_NU; break;
             case P_EN: nt = (long long)z.nE; cost = (double)p.RES_NU; break;
             case P_CHECK: nt = (std::max(std::max(z.n4, z.nT), z.nE) + 255) / 256; break;
@@ -316,22 +325,64 @@ void build_tiles(const std::vector<atmo_params> &ps, const atmo_options &o,
     }
 }
 
-atmo_status launch_all(Plan &pl, hipStream_t st) {
+// Event pairs around each pass launch (ATMO_FLAG_TIME_PASSES).
+struct PassTimer {
+    struct Rec {
+        int pass, order;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    hipStream_t st = nullptr;
+    bool on = false;
+    ~PassTimer() {
+        for (auto &r : recs) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+    }
+    hipError_t begin(int pass, int order) {
+        if (!on) return hipSuccess;
+        Rec r{pass, order, nullptr, nullptr};
+        hipError_t e = hipEventCreate(&r.a);
+        if (e == hipSuccess) e = hipEventCreate(&r.b);
+        if (e == hipSuccess) e = hipEventRecord(r.a, st);
+        recs.push_back(r);
+        return e;
+    }
+    hipError_t end() { return on ? hipEventRecord(recs.back().b, st) : hipSuccess; }
+};
+
+atmo_status launch_all(Plan &pl, hipStream_t st, PassTimer *tm = nullptr) {
+    PassTimer none;
+    PassTimer &t = tm ? *tm : none;
+    t.st = st;
     const SetDesc *S = (const SetDesc *)pl.dsets.p;
     const uint2 *T = (const uint2 *)pl.dtiles.p;
     const atmo_options &o = pl.opts;
     const char *dirs = (const char *)pl.ddirs.p;
+    HIPCHK(t.begin(0, 1));
     HIPCHK(launch_transmittance(S, T + pl.off[P_T], pl.cnt[P_T], o.transmittance_samples, st));
+    HIPCHK(t.end());
+    HIPCHK(t.begin(1, 1));
     HIPCHK(launch_irradiance1(S, T + pl.off[P_E1], pl.cnt[P_E1], st));
+    HIPCHK(t.end());
+    HIPCHK(t.begin(2, 1));
     HIPCHK(launch_inscatter1(S, T + pl.off[P_S1], pl.cnt[P_S1], o.inscatter_samples, st));
+    HIPCHK(t.end());
     for (int order = 2; order <= o.orders; ++order) {
         const bool first = order == 2;
+        HIPCHK(t.begin(3, order));
         HIPCHK(launch_inscatterS(S, T + pl.off[P_J], pl.cnt[P_J], o.spherical_samples,
                                  dirs + pl.dirJT_off, dirs + pl.dirJW_off, first,
                                  first ? pl.ldsJ1 : pl.ldsJ, st));
+        HIPCHK(t.end());
+        HIPCHK(t.begin(4, order));
         HIPCHK(launch_irradianceN(S, T + pl.off[P_EN], pl.cnt[P_EN], o.irradiance_samples,
                                   dirs + pl.dirE_off, first, first ? pl.ldsE1 : pl.ldsEN, st));
+        HIPCHK(t.end());
+        HIPCHK(t.begin(5, order));
         HIPCHK(launch_inscatterN(S, T + pl.off[P_SN], pl.cnt[P_SN], o.inscatter_samples, st));
+        HIPCHK(t.end());
     }
     return ATMO_OK;
 }
@@ -470,6 +521,15 @@ atmo_status atmo_engine_destroy(atmo_engine *e) {
     return ATMO_OK;
 }
 
+atmo_status atmo_engine_pass_times(atmo_engine *e, atmo_pass_time *out, int capacity, int *count) {
+    if (!e || !count || capacity < 0 || (capacity > 0 && !out))
+        return fail(ATMO_ERR_INVALID_ARGUMENT, "bad arguments");
+    int n = std::min<int>(capacity, (int)e->times.size());
+    for (int i = 0; i < n; ++i) out[i] = e->times[i];
+    *count = (int)e->times.size();
+    return ATMO_OK;
+}
+
 atmo_status atmo_engine_workspace_size(const atmo_params *sets, int nsets, const atmo_options *opts,
                                        size_t *bytes) {
     if (!sets || nsets <= 0 || !bytes) return fail(ATMO_ERR_INVALID_ARGUMENT, "bad arguments");
@@ -516,7 +576,18 @@ atmo_status atmo_engine_precompute(atmo_engine *e, const atmo_params *sets, int 
         HIPCHK(hipStreamSynchronize(hs));
         HIPCHK(hipMemcpy(pl.dsets.p, pl.hsets.data(), nsets * sizeof(SetDesc), hipMemcpyHostToDevice));
     }
-    if (o.flags & ATMO_FLAG_NO_GRAPH) {
+    if (o.flags & ATMO_FLAG_TIME_PASSES) {
+        PassTimer tm;
+        tm.on = true;
+        if ((st = launch_all(pl, hs, &tm)) != ATMO_OK) return st;
+        HIPCHK(hipStreamSynchronize(hs));
+        e->times.clear();
+        for (auto &r : tm.recs) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+            e->times.push_back({r.pass, r.order, ms});
+        }
+    } else if (o.flags & ATMO_FLAG_NO_GRAPH) {
         if ((st = launch_all(pl, hs)) != ATMO_OK) return st;
     } else {
         if (!pl.gexec) {

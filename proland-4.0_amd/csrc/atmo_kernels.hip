@@ -174,11 +174,11 @@ __device__ __forceinline__ void stage_slice(const SetDesc &s, const float4 *cons
     const size_t W = s.W4, HW = (size_t)s.RMU * s.W4;
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
-        const float4 *tb = tabs[n];
-        float3 c000 = f3(tb[z0 * HW + y0 * W + x0]), c001 = f3(tb[z0 * HW + y0 * W + x1]);
-        float3 c010 = f3(tb[z0 * HW + y1 * W + x0]), c011 = f3(tb[z0 * HW + y1 * W + x1]);
-        float3 c100 = f3(tb[z1 * HW + y0 * W + x0]), c101 = f3(tb[z1 * HW + y0 * W + x1]);
-        float3 c110 = f3(tb[z1 * HW + y1 * W + x0]), c111 = f3(tb[z1 * HW + y1 * W + x1]);
+        const gfloat4 *tb = gptr(tabs[n]);
+        float3 c000 = ld3(tb + z0 * HW + y0 * W + x0), c001 = ld3(tb + z0 * HW + y0 * W + x1);
+        float3 c010 = ld3(tb + z0 * HW + y1 * W + x0), c011 = ld3(tb + z0 * HW + y1 * W + x1);
+        float3 c100 = ld3(tb + z1 * HW + y0 * W + x0), c101 = ld3(tb + z1 * HW + y0 * W + x1);
+        float3 c110 = ld3(tb + z1 * HW + y1 * W + x0), c111 = ld3(tb + z1 * HW + y1 * W + x1);
         float3 v = lerp3(lerp3(lerp3(c000, c001, ax), lerp3(c010, c011, ax), ay),
                          lerp3(lerp3(c100, c101, ax), lerp3(c110, c111, ax), ay), az);
         out[n] = make_float4(v.x, v.y, v.z, 0.f);
@@ -189,10 +189,21 @@ __device__ __forceinline__ void stage_slice(const SetDesc &s, const float4 *cons
 // Pass dJn — inscatterS.glsl:111-189.  One workgroup per (set, r layer,
 // mu_s column); its RMU x RNU texels share r and mu_s, hence the staged
 // slice values.  FIRST = order 2 (Rayleigh and Mie stored separately).
+//
+// Each thread integrates G texels of one mu row (G nu slices): they share
+// the view vector v, so the phase factors of nu2 = v.w are evaluated once per
+// direction for all G.  Per direction and texel the lookup position is
+// pos = (nu1 + 1)/2 (RNU - 1) + 1 = posA s'.w + posB, with s' = posA s folded
+// in per texel and the mu_s term folded in per theta.
 // ---------------------------------------------------------------------------
-struct DirJ {      // per-theta constants of the 16x32 (NSx2NS) grid
+struct DirJ {      // per-theta constants of the NS x 2NS grid
     float ct, st, dw, pad;
 };
+
+#ifndef ATMO_JG
+#define ATMO_JG 4
+#endif
+constexpr int kJG = ATMO_JG;  // texels per thread in k_inscatterS
 
 template <bool FIRST>
 __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ sets,
@@ -200,6 +211,7 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
                                                     const DirJ *__restrict__ dirT,
                                                     const float2 *__restrict__ dirW) {
     constexpr int NT = FIRST ? 2 : 1;
+    constexpr int G = kJG;
     extern __shared__ float4 lds[];
     const uint2 tl = tiles[blockIdx.x];
     const SetDesc &s = sets[tl.x];
@@ -215,10 +227,11 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
 
     const LayerGeom L = s.layers[l];
     const double r = fmin(fmax(L.r, s.Rg), s.Rt);          // inscatterS.glsl:112
-    double muS = fmin(fmax(texel_mus(s, m), -1.0), 1.0);   // :114
+    const double muS = fmin(fmax(texel_mus(s, m), -1.0), 1.0);   // :114
     const double rho2 = r * r - s.Rg2;
     const double txs = tex4d_umus(s, muS) * s.RMUS - 0.5;
     const float4 *tabs[2] = {s.dSR4, s.dSM4};
+    const double posAd = 0.5 * (NU - 1), posBd = posAd + 1.0;
 
     // (1) slice values V[it][kk], kk = slice + 1
     for (int e = threadIdx.x; e < NS * NV; e += blockDim.x) {
@@ -243,12 +256,14 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
         epair[2 * k] = a;
         epair[2 * k + 1] = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.f);
     }
-    // (3) ground visibility and ground-to-x transmittance per theta (:129-138)
+    // (3) per theta: ground visibility, ground-to-x transmittance (:129-138), and
+    // the theta-uniform parts of the lookup positions
     for (int it = threadIdx.x; it < NS; it += blockDim.x) {
         const double ct = cos((it + 0.5) * (kPI / NS));
         const double q = s.Rg / r;
         const double cthetamin = -sqrt(1.0 - q * q);
-        float4 g0 = make_float4(0.f, 0.f, 0.f, 0.f), g1 = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 g0 = make_float4(0.f, 0.f, 0.f, 0.f);
+        double c1 = 0.0, c0 = 0.0;
         if (ct < cthetamin) {
             const double dg = -r * ct - sqrt(r * r * (ct * ct - 1.0) + s.Rg2);
             const double mug = -(r * ct + dg) / s.Rg;
@@ -267,13 +282,15 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
             }
             const float gr = (float)(s.alpha / kPI);
             g0 = make_float4(gr * tr.x, gr * tr.y, gr * tr.z, 1.0f);
-            // E texel coordinate: ((r muS + dg nu1)/Rg + 0.2)/1.2 * SW - 0.5 = c1 nu1 + c0
+            // E texel coordinate ((r muS + dg nu1)/Rg + 0.2)/1.2 SW - 0.5 as c1 pos + c0
             const double sc = s.SW / 1.2;
-            g1 = make_float4((float)(dg / s.Rg * sc), (float)((r * muS / s.Rg + 0.2) * sc - 0.5),
-                             0.f, 0.f);
+            const double e1 = dg / s.Rg * sc, e0 = (r * muS / s.Rg + 0.2) * sc - 0.5;
+            c1 = e1 / posAd;
+            c0 = e0 - e1 * posBd / posAd;
         }
         gdat[2 * it] = g0;
-        gdat[2 * it + 1] = g1;
+        gdat[2 * it + 1] = make_float4((float)c1, (float)c0, (float)(posAd * muS * ct + posBd),
+                                       (float)((kPI / NS) * (kPI / NS) * sin((it + 0.5) * (kPI / NS))));
     }
     __syncthreads();
     // (4) pairs {V[kk-1], V[kk] - V[kk-1]}
@@ -293,81 +310,115 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
     const float eR = (float)exp(-hr / s.HR), eM = (float)exp(-hr / s.HM);
     const float A0 = s.bR[0] * eR * s.kR, A1 = s.bR[1] * eR * s.kR, A2 = s.bR[2] * eR * s.kR;
     const float B0 = s.bMS[0] * eM * s.kM, B1 = s.bMS[1] * eM * s.kM, B2 = s.bMS[2] * eM * s.kM;
-    const float posA = 0.5f * (NU - 1), posB = 0.5f * (NU - 1) + 1.0f;
+    const float iposA = (float)(1.0 / posAd), nposBi = (float)(-posBd / posAd);
     const float g2p1 = s.g2p1, m2g = s.m2g;
     const float SWm1 = (float)(s.SW - 1);
-    const double muSmu_base = muS;
+    const unsigned NUu = (unsigned)NU;
+    const int ngk = (NU + G - 1) / G;
+    const int ngroups = s.RMU * ngk;
 
-    const int ntex = s.RMU * NU;
-    for (int tt = threadIdx.x; tt < ntex; tt += blockDim.x) {
-        const int y = tt % s.RMU, k = tt / s.RMU;
-        const int x = k * s.RMUS + m;
-        double mu, muS_t, nu;
-        mu_mus_nu(s, L, x, y, mu, muS_t, nu);
-        // inscatterS.glsl:113-122 (FP64)
-        mu = fmin(fmax(mu, -1.0), 1.0);
-        const double mS = muSmu_base;
-        const double var = sqrt(1.0 - mu * mu) * sqrt(1.0 - mS * mS);
-        nu = fmin(fmax(nu, mS * mu - var), mS * mu + var);
-        const double vxd = sqrt(1.0 - mu * mu);
-        const double sxd = vxd == 0.0 ? 0.0 : (nu - mS * mu) / vxd;
-        const double syd = sqrt(fmax(0.0, 1.0 - sxd * sxd - mS * mS));
-        const float vx = (float)vxd, vz = (float)mu;
-        const float sx = (float)sxd, sy = (float)syd, sz = (float)mS;
+    for (int grp = threadIdx.x; grp < ngroups; grp += blockDim.x) {
+        const int y = grp % s.RMU, kb = (grp / s.RMU) * G;
+        float vx = 0.f, vz = 0.f;
+        float sxp[G], syp[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int k = min(kb + g, NU - 1);
+            double mu, muS_t, nu;
+            mu_mus_nu(s, L, k * s.RMUS + m, y, mu, muS_t, nu);
+            // inscatterS.glsl:113-122 (FP64)
+            mu = fmin(fmax(mu, -1.0), 1.0);
+            const double var = sqrt(1.0 - mu * mu) * sqrt(1.0 - muS * muS);
+            nu = fmin(fmax(nu, muS * mu - var), muS * mu + var);
+            const double vxd = sqrt(1.0 - mu * mu);
+            const double sxd = vxd == 0.0 ? 0.0 : (nu - muS * mu) / vxd;
+            const double syd = sqrt(fmax(0.0, 1.0 - sxd * sxd - muS * muS));
+            vx = (float)vxd;
+            vz = (float)mu;
+            sxp[g] = (float)(posAd * sxd);
+            syp[g] = (float)(posAd * syd);
+        }
+        const float cx1 = m2g * vx;
+        float aR[G][3], aM[G][3];
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) aR[g][c] = aM[g][c] = 0.f;
 
-        float r0 = 0.f, r1 = 0.f, r2 = 0.f;
         for (int it = 0; it < NS; ++it) {
-            const DirJ dt = dirT[it];
-            const float szwz = sz * dt.ct, vzwz = vz * dt.ct;
+            const float ct = dirT[it].ct;
             const float4 g0 = gdat[2 * it], g1 = gdat[2 * it + 1];
             const bool ground = g0.w != 0.0f;
+            const float vzwz = vz * ct;
+            const float cx0 = fmaf(m2g, vzwz, g2p1);
+            const float dw = g1.w, cp = g1.z;
             const float4 *pt = pairs + (size_t)it * NP * NT * 2;
             const float2 *wrow = dirW + (size_t)it * 2 * NS;
-            float aR0 = 0.f, aR1 = 0.f, aR2 = 0.f, aM0 = 0.f, aM1 = 0.f, aM2 = 0.f;
+            float2 wn = wrow[0];
             for (int ip = 0; ip < 2 * NS; ++ip) {
-                const float2 w = wrow[ip];
-                const float nu1 = fmaf(sx, w.x, fmaf(sy, w.y, szwz));
+                const float2 w = wn;
+                if (ip + 1 < 2 * NS) wn = wrow[ip + 1];  // scalar prefetch of the next direction
+                // phase factors of nu2 = v.w, shared by the G texels (dw folded in)
                 const float nu2 = fmaf(vx, w.x, vzwz);
-                float pos = fmaf(nu1, posA, posB);
-                float fl = floorf(pos);
-                float f = pos - fl;
-                int kk = min(max((int)fl, 0), NU);
-                const float4 *pp = pt + (size_t)kk * NT * 2;
-                float4 a = pp[0], d = pp[1];
-                float l0 = fmaf(f, d.x, a.x), l1 = fmaf(f, d.y, a.y), l2 = fmaf(f, d.z, a.z);
-                if (FIRST) {
-                    float4 am = pp[2], dm = pp[3];
-                    float m0 = fmaf(f, dm.x, am.x), m1 = fmaf(f, dm.y, am.y), m2 = fmaf(f, dm.z, am.z);
-                    const float q1 = fmaf(nu1, nu1, 1.0f);
-                    const float y1 = rsqrtf(fmaf(m2g, nu1, g2p1));
-                    const float t1 = q1 * (y1 * y1 * y1);
-                    l0 = fmaf(l0, q1, m0 * t1);
-                    l1 = fmaf(l1, q1, m1 * t1);
-                    l2 = fmaf(l2, q1, m2 * t1);
+                const float q2 = fmaf(nu2 * dw, nu2, dw);
+                const float y2 = __builtin_amdgcn_rsqf(fmaf(cx1, w.x, cx0));
+                const float t2 = q2 * (y2 * y2 * y2);
+                float pos[G], lv[G][3];
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    pos[g] = fmaf(sxp[g], w.x, fmaf(syp[g], w.y, cp));
+                    const unsigned kk = min(__float2uint_rz(pos[g]), NUu);
+                    const float f = __builtin_amdgcn_fractf(pos[g]);
+                    const float4 *pp = pt + kk * (NT * 2);
+                    const float4 a = pp[0], d = pp[1];
+                    lv[g][0] = fmaf(f, d.x, a.x);
+                    lv[g][1] = fmaf(f, d.y, a.y);
+                    lv[g][2] = fmaf(f, d.z, a.z);
+                    if (FIRST) {
+                        const float4 am = pp[2], dm = pp[3];
+                        const float m0 = fmaf(f, dm.x, am.x), m1 = fmaf(f, dm.y, am.y),
+                                    m2 = fmaf(f, dm.z, am.z);
+                        const float nu1 = fmaf(pos[g], iposA, nposBi);
+                        const float q1 = fmaf(nu1, nu1, 1.0f);
+                        const float y1 = __builtin_amdgcn_rsqf(fmaf(m2g, nu1, g2p1));
+                        const float t1 = q1 * (y1 * y1 * y1);
+                        lv[g][0] = fmaf(lv[g][0], q1, m0 * t1);
+                        lv[g][1] = fmaf(lv[g][1], q1, m1 * t1);
+                        lv[g][2] = fmaf(lv[g][2], q1, m2 * t1);
+                    }
                 }
                 if (ground) {
-                    float xg = fmaf(nu1, g1.x, g1.y);
-                    xg = fminf(fmaxf(xg, 0.0f), SWm1);
-                    float fg = floorf(xg);
-                    float gf = xg - fg;
-                    int kg = (int)fg;
-                    float4 ea = epair[2 * kg], ed = epair[2 * kg + 1];
-                    l0 = fmaf(g0.x, fmaf(gf, ed.x, ea.x), l0);
-                    l1 = fmaf(g0.y, fmaf(gf, ed.y, ea.y), l1);
-                    l2 = fmaf(g0.z, fmaf(gf, ed.z, ea.z), l2);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const float xg = __builtin_amdgcn_fmed3f(fmaf(pos[g], g1.x, g1.y), 0.0f, SWm1);
+                        const unsigned kg = __float2uint_rz(xg);
+                        const float fg = __builtin_amdgcn_fractf(xg);
+                        const float4 ea = epair[2 * kg], ed = epair[2 * kg + 1];
+                        lv[g][0] = fmaf(g0.x, fmaf(fg, ed.x, ea.x), lv[g][0]);
+                        lv[g][1] = fmaf(g0.y, fmaf(fg, ed.y, ea.y), lv[g][1]);
+                        lv[g][2] = fmaf(g0.z, fmaf(fg, ed.z, ea.z), lv[g][2]);
+                    }
                 }
-                const float q2 = fmaf(nu2, nu2, 1.0f);
-                const float y2 = rsqrtf(fmaf(m2g, nu2, g2p1));
-                const float t2 = q2 * (y2 * y2 * y2);
-                aR0 = fmaf(l0, q2, aR0); aR1 = fmaf(l1, q2, aR1); aR2 = fmaf(l2, q2, aR2);
-                aM0 = fmaf(l0, t2, aM0); aM1 = fmaf(l1, t2, aM1); aM2 = fmaf(l2, t2, aM2);
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        aR[g][c] = fmaf(lv[g][c], q2, aR[g][c]);
+                        aM[g][c] = fmaf(lv[g][c], t2, aM[g][c]);
+                    }
+                }
             }
-            r0 = fmaf(dt.dw, fmaf(A0, aR0, B0 * aM0), r0);
-            r1 = fmaf(dt.dw, fmaf(A1, aR1, B1 * aM1), r1);
-            r2 = fmaf(dt.dw, fmaf(A2, aR2, B2 * aM2), r2);
         }
-        const size_t o = ((size_t)l * s.RMU + y) * s.W4 + x;
-        s.dJ4[o] = make_float4(r0, r1, r2, 0.f);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int k = kb + g;
+            if (k < NU) {
+                const size_t o = ((size_t)l * s.RMU + y) * s.W4 + (size_t)k * s.RMUS + m;
+                s.dJ4[o] = make_float4(fmaf(A0, aR[g][0], B0 * aM[g][0]),
+                                       fmaf(A1, aR[g][1], B1 * aM[g][1]),
+                                       fmaf(A2, aR[g][2], B2 * aM[g][2]), 0.f);
+            }
+        }
     }
 }
 
@@ -494,108 +545,205 @@ __global__ __launch_bounds__(256) void k_irradianceN(const SetDesc *__restrict__
 
 // ---------------------------------------------------------------------------
 // Pass dSn — inscatterN.glsl:105-131 fused with copyInscatterN.glsl:105-110 +
-// ADD blend (S.rgb += dS / phaseR(nu)).  One thread per 4D texel.
+// ADD blend (S.rgb += dS / phaseR(nu)).
+//
+// The view ray (r, mu) of a texel depends only on its layer and mu row, so
+// every quantity of an integration sample that does not involve the sun
+// (sample radius and zenith cosine, the texture4D r and mu coordinates and
+// filter weights, the transmittance T(r, mu, t)) is shared by the
+// RNU x RMU_S texels of the row.  A workgroup covers up to 4 rows: it stages
+// those per-sample terms in LDS (FP64 geometry, in chunks of 64 samples) and
+// each thread then integrates its texels with only the mu_s-dependent work.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float3 trilinear(const float4 *__restrict__ tb, const SetDesc &s,
-                                            float tx, float ty, float tz) {
-    int z0, z1, y0, y1, x0, x1;
-    float az, ay, ax;
-    gl_axis(tx, s.W4, x0, x1, ax);
-    gl_axis(ty, s.RMU, y0, y1, ay);
-    gl_axis(tz, s.RR, z0, z1, az);
-    const int W = s.W4, HW = s.RMU * s.W4;
-    float3 c000 = f3(tb[z0 * HW + y0 * W + x0]), c001 = f3(tb[z0 * HW + y0 * W + x1]);
-    float3 c010 = f3(tb[z0 * HW + y1 * W + x0]), c011 = f3(tb[z0 * HW + y1 * W + x1]);
-    float3 c100 = f3(tb[z1 * HW + y0 * W + x0]), c101 = f3(tb[z1 * HW + y0 * W + x1]);
-    float3 c110 = f3(tb[z1 * HW + y1 * W + x0]), c111 = f3(tb[z1 * HW + y1 * W + x1]);
-    return lerp3(lerp3(lerp3(c000, c001, ax), lerp3(c010, c011, ax), ay),
-                 lerp3(lerp3(c100, c101, ax), lerp3(c110, c111, ax), ay), az);
-}
+constexpr int kNChunk = 64;   // samples staged per chunk
+constexpr int kNRows = 4;     // max mu rows per workgroup
+
+__device__ __forceinline__ int sn_rows_per_wg(int W4) { return max(1, min(kNRows, 256 / W4)); }
+
+struct SnEntry {
+    float4 w;     // bilinear (r, mu) weights of the 4 table rows
+    int4 base;    // offsets of the 4 rows (z0y0, z0y1, z1y0, z1y1)
+    float4 tw;    // T(r, mu, t) * trapezoid weight (rgb), t
+    float4 ir;    // 1 / r_i
+};
 
 __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ sets,
                                                     const uint2 *__restrict__ tiles,
                                                     int nsamp) {
+    __shared__ SnEntry ent[kNRows][kNChunk];
+    __shared__ float rowdx[kNRows];
     const uint2 tl = tiles[blockIdx.x];
     const SetDesc &s = sets[tl.x];
-    const int t = (int)tl.y * 256 + threadIdx.x;
-    if (t >= s.n4) return;
-    const int x = t % s.W4;
-    const int y = (t / s.W4) % s.RMU;
-    const int l = t / (s.W4 * s.RMU);
+    const int RPW = sn_rows_per_wg(s.W4);
+    const int ngy = (s.RMU + RPW - 1) / RPW;
+    const int l = (int)tl.y / ngy;
+    const int yb = ((int)tl.y % ngy) * RPW;
+    const int nrows = min(RPW, s.RMU - yb);
     const LayerGeom L = s.layers[l];
-    double mu, muS, nu;
-    mu_mus_nu(s, L, x, y, mu, muS, nu);
     const double r = L.r;
-    const double rmu = r * mu;
     const double rho2 = r * r - s.Rg2;
-    const double delta = rmu * rmu - rho2;                 // constant along the ray
-    const bool gnd_ray = delta > 0.0;
-    const double sqd = gnd_ray ? sqrt(delta) : 0.0;
-    const double sqsky = sqrt(fmax(delta + s.H * s.H, 0.0));
-    const double dx = limit_d(s, r, mu) / nsamp;
-    const bool up = mu > 0.0;
-    const float3 Tfix = T_lookup(s, (float)(r - s.Rg), (float)(up ? mu : -mu));
-    const double muSr = muS * r;
-    // nu lies on a slice of the table: the nu lerp of texture4D has weight 0 or 1
-    double pos = (nu + 1.0) / 2.0 * (s.RNU - 1.0);
-    double kr = rint(pos);
-    if (fabs(pos - kr) < 1e-9) pos = kr;
-    const double kfl = floor(pos);
-    const float fk = (float)(pos - kfl);
-    const int k0 = (int)kfl;
-    const float fH = s.f_H, fRR1 = (float)(s.RR - 1), fMU = (float)s.RMU;
-    const float cG = 0.5f - 0.5f / s.RMU, cS = 0.5f + 0.5f / s.RMU, cM = 0.5f - 1.0f / s.RMU;
-    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-    for (int i = 0; i <= nsamp; ++i) {
-        const double ti = i * dx;
-        const double q = rho2 + ti * (ti + 2.0 * rmu);  // ri^2 - Rg^2
-        const double rmui = rmu + ti;                    // ri * mui
-        const float ri = sqrtf((float)(q + s.Rg2));
-        const float rho = sqrtf(fmaxf((float)q, 0.0f));
-        // texture4D(dJ, ri, mui, muSi, nu) coordinates
-        float uMu;
-        if (rmui < 0.0 && gnd_ray) {
-            uMu = fmaf(-rho / (float)(sqd - rmui), cM, cG);
-        } else {
-            uMu = fmaf((float)(sqsky - rmui) / (rho + fH), cM, cS);
+    const int HW = s.RMU * s.W4;
+
+    // texels of this thread: rows yb.. (row-major, x fastest)
+    const int per = nrows * s.W4;
+    const int nmine = (per + (int)blockDim.x - 1) / (int)blockDim.x;
+    constexpr int MAXT = 4;   // W4 <= 1024 (validated on the host)
+    float acc[MAXT][3];
+    float nuf[MAXT], muSr[MAXT], fk[MAXT];
+    int k0[MAXT], rowi[MAXT];
+#pragma unroll
+    for (int j = 0; j < MAXT; ++j) {
+        acc[j][0] = acc[j][1] = acc[j][2] = 0.f;
+        nuf[j] = muSr[j] = fk[j] = 0.f;
+        k0[j] = rowi[j] = 0;
+        const int e = threadIdx.x + j * (int)blockDim.x;
+        if (j < nmine && e < per) {
+            const int row = e / s.W4, x = e % s.W4;
+            double mu, muS, nu;
+            mu_mus_nu(s, L, x, yb + row, mu, muS, nu);
+            // nu lies on a slice of the table: the nu lerp of texture4D has weight 0 or 1
+            double pos = (nu + 1.0) / 2.0 * (s.RNU - 1.0);
+            const double kr = rint(pos);
+            if (fabs(pos - kr) < 1e-9) pos = kr;
+            const double kfl = floor(pos);
+            fk[j] = (float)(pos - kfl);
+            k0[j] = (int)kfl;
+            nuf[j] = (float)nu;
+            muSr[j] = (float)(muS * r);
+            rowi[j] = row;
         }
-        const float ty = uMu * fMU - 0.5f;
-        const float tz = rho / fH * fRR1;
-        const float muSi = (float)fma(nu, ti, muSr) / ri;
-        const float txs = tex4d_txs_f(s, muSi);
-        float3 J = trilinear(s.dJ4, s, (float)k0 * s.RMUS + txs, ty, tz);
-        if (fk != 0.0f) {
-            float3 J1 = trilinear(s.dJ4, s, (float)(k0 + 1) * s.RMUS + txs, ty, tz);
-            J = lerp3(J, J1, fk);
-        }
-        // transmittance(r, mu, ti) (common.glsl:272-282)
-        const float h1 = (float)q / (ri + s.f_Rg);
-        const float mu1 = (float)rmui / ri;
-        float3 Tseg;
-        if (up) {
-            float3 b = T_lookup(s, h1, mu1);
-            Tseg = make_float3(fminf(Tfix.x / b.x, 1.f), fminf(Tfix.y / b.y, 1.f),
-                               fminf(Tfix.z / b.z, 1.f));
-        } else {
-            float3 a = T_lookup(s, h1, -mu1);
-            Tseg = make_float3(fminf(a.x / Tfix.x, 1.f), fminf(a.y / Tfix.y, 1.f),
-                               fminf(a.z / Tfix.z, 1.f));
-        }
-        const float w = (i == 0 || i == nsamp) ? 0.5f : 1.0f;
-        a0 = fmaf(w * J.x, Tseg.x, a0);
-        a1 = fmaf(w * J.y, Tseg.y, a1);
-        a2 = fmaf(w * J.z, Tseg.z, a2);
     }
-    const float fdx = (float)dx;
-    const float4 o = make_float4(a0 * fdx, a1 * fdx, a2 * fdx, 0.f);
-    s.dSR4[t] = o;
-    if (s.S4) {
-        const float pr = (float)((3.0 / (16.0 * kPI)) * (1.0 + nu * nu));
-        float4 S = s.S4[t];
-        S.x += o.x / pr;
-        S.y += o.y / pr;
-        S.z += o.z / pr;
-        s.S4[t] = S;
+
+    for (int c0 = 0; c0 <= nsamp; c0 += kNChunk) {
+        const int nc = min(kNChunk, nsamp + 1 - c0);
+        __syncthreads();
+        // stage the row-uniform terms of samples c0 .. c0+nc-1 (FP64 geometry)
+        for (int e = threadIdx.x; e < nrows * kNChunk; e += blockDim.x) {
+            const int row = e / kNChunk, js = e % kNChunk;
+            if (js >= nc) continue;
+            const int i = c0 + js;
+            double mu, muS_u, nu_u;
+            mu_mus_nu(s, L, 0, yb + row, mu, muS_u, nu_u);
+            const double rmu = r * mu;
+            const double delta = rmu * rmu - rho2;            // constant along the ray
+            const double dx = limit_d(s, r, mu) / nsamp;
+            if (i == 0) rowdx[row] = (float)dx;
+            const double ti = i * dx;
+            const double q = rho2 + ti * (ti + 2.0 * rmu);   // ri^2 - Rg^2
+            const double rmui = rmu + ti;                     // ri * mui
+            const double ri = sqrt(q + s.Rg2);
+            const double rho = sqrt(fmax(q, 0.0));
+            double f, c;
+            if (rmui < 0.0 && delta > 0.0) {
+                f = -rho / (sqrt(delta) - rmui);
+                c = 0.5 - 0.5 / s.RMU;
+            } else {
+                f = (sqrt(fmax(delta + s.H * s.H, 0.0)) - rmui) / (rho + s.H);
+                c = 0.5 + 0.5 / s.RMU;
+            }
+            const double ty = (c + f * (0.5 - 1.0 / s.RMU)) * s.RMU - 0.5;
+            const double tz = rho / s.H * (s.RR - 1.0);
+            int z0, z1, y0, y1;
+            float az, ay;
+            gl_axis_d(tz, s.RR, z0, z1, az);
+            gl_axis_d(ty, s.RMU, y0, y1, ay);
+            // transmittance(r, mu, ti) (common.glsl:272-282)
+            const float h1 = (float)(q / (ri + s.Rg));
+            const float mu1 = (float)(rmui / ri);
+            float3 Tseg;
+            if (mu > 0.0) {
+                const float3 a = T_lookup(s, (float)(r - s.Rg), (float)mu), b = T_lookup(s, h1, mu1);
+                Tseg = make_float3(fminf(a.x / b.x, 1.f), fminf(a.y / b.y, 1.f), fminf(a.z / b.z, 1.f));
+            } else {
+                const float3 a = T_lookup(s, h1, -mu1), b = T_lookup(s, (float)(r - s.Rg), (float)-mu);
+                Tseg = make_float3(fminf(a.x / b.x, 1.f), fminf(a.y / b.y, 1.f), fminf(a.z / b.z, 1.f));
+            }
+            const float wt = (i == 0 || i == nsamp) ? 0.5f : 1.0f;
+            SnEntry en;
+            en.w = make_float4((1.f - az) * (1.f - ay), (1.f - az) * ay, az * (1.f - ay), az * ay);
+            en.base = make_int4((z0 * HW + y0 * s.W4) * 16, (z0 * HW + y1 * s.W4) * 16,
+                                (z1 * HW + y0 * s.W4) * 16, (z1 * HW + y1 * s.W4) * 16);
+            en.tw = make_float4(Tseg.x * wt, Tseg.y * wt, Tseg.z * wt, (float)ti);
+            en.ir = make_float4((float)(1.0 / ri), 0.f, 0.f, 0.f);
+            ent[row][js] = en;
+        }
+        __syncthreads();
+        typedef __attribute__((address_space(1))) const char gchar;
+        gchar *J = (gchar *)s.dJ4;
+        const int W4m1 = s.W4 - 1;
+        const unsigned W4m2 = (unsigned)(s.W4 - 2);
+#pragma unroll
+        for (int j = 0; j < MAXT; ++j) {
+            if (j >= nmine) break;
+            const int e = threadIdx.x + j * (int)blockDim.x;
+            if (e >= per) break;
+            const SnEntry *er = ent[rowi[j]];
+            const float kbase = (float)(k0[j] * s.RMUS);
+            float a0 = acc[j][0], a1 = acc[j][1], a2 = acc[j][2];
+            for (int js = 0; js < nc; ++js) {
+                const SnEntry en = er[js];
+                const float muSi = fmaf(nuf[j], en.tw.w, muSr[j]) * en.ir.x;
+                // x: texel column; clamped to [0, W4-1] and split so that i0 + 1 <= W4 - 1
+                float tx = __builtin_amdgcn_fmed3f(kbase + tex4d_txs_f(s, muSi), 0.0f, (float)W4m1);
+                const unsigned i0 = min(__float2uint_rz(tx), W4m2);
+                const float ax = tx - (float)i0;
+                float3 v = make_float3(0.f, 0.f, 0.f);
+                const float wr[4] = {en.w.x, en.w.y, en.w.z, en.w.w};
+                const unsigned br[4] = {(unsigned)en.base.x, (unsigned)en.base.y, (unsigned)en.base.z,
+                                        (unsigned)en.base.w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const unsigned off = br[q] + i0 * 16u;
+                    const float3 c0v = ld3((const gfloat4 *)(J + off));
+                    const float3 c1v = ld3((const gfloat4 *)(J + off + 16u));
+                    const float wq = wr[q], wa = wr[q] * ax;
+                    v.x = fmaf(wq, c0v.x, fmaf(wa, c1v.x - c0v.x, v.x));
+                    v.y = fmaf(wq, c0v.y, fmaf(wa, c1v.y - c0v.y, v.y));
+                    v.z = fmaf(wq, c0v.z, fmaf(wa, c1v.z - c0v.z, v.z));
+                }
+                if (fk[j] != 0.0f) {  // off-grid nu: blend with the next slice
+                    const float tx2 = __builtin_amdgcn_fmed3f(tx + (float)s.RMUS, 0.0f, (float)W4m1);
+                    const unsigned j0 = min(__float2uint_rz(tx2), W4m2);
+                    const float bx = tx2 - (float)j0;
+                    float3 v2 = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const unsigned off = br[q] + j0 * 16u;
+                        const float3 c0v = ld3((const gfloat4 *)(J + off));
+                        const float3 c1v = ld3((const gfloat4 *)(J + off + 16u));
+                        const float wq = wr[q], wa = wr[q] * bx;
+                        v2.x = fmaf(wq, c0v.x, fmaf(wa, c1v.x - c0v.x, v2.x));
+                        v2.y = fmaf(wq, c0v.y, fmaf(wa, c1v.y - c0v.y, v2.y));
+                        v2.z = fmaf(wq, c0v.z, fmaf(wa, c1v.z - c0v.z, v2.z));
+                    }
+                    v = lerp3(v, v2, fk[j]);
+                }
+                a0 = fmaf(v.x, en.tw.x, a0);
+                a1 = fmaf(v.y, en.tw.y, a1);
+                a2 = fmaf(v.z, en.tw.z, a2);
+            }
+            acc[j][0] = a0; acc[j][1] = a1; acc[j][2] = a2;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < MAXT; ++j) {
+        if (j >= nmine) break;
+        const int e = threadIdx.x + j * (int)blockDim.x;
+        if (e >= per) break;
+        const int row = e / s.W4, x = e % s.W4;
+        const size_t t = ((size_t)l * s.RMU + yb + row) * s.W4 + x;
+        const float dx = rowdx[row];
+        const float4 o = make_float4(acc[j][0] * dx, acc[j][1] * dx, acc[j][2] * dx, 0.f);
+        s.dSR4[t] = o;
+        if (s.S4) {
+            const double nu = nuf[j];
+            const float pr = (float)((3.0 / (16.0 * kPI)) * (1.0 + nu * nu));
+            float4 S = s.S4[t];
+            S.x += o.x / pr;
+            S.y += o.y / pr;
+            S.z += o.z / pr;
+            s.S4[t] = S;
+        }
     }
 }
 

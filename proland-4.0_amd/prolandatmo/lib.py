@@ -10,7 +10,7 @@ import os
 from .params import AtmoOptions, AtmoParams, options as default_options, params_array
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "libproland_atmo.so")
+LIB_PATH = os.environ.get("PROLAND_ATMO_LIB") or os.path.join(PKG_DIR, "libproland_atmo.so")
 
 ATMO_OK = 0
 STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "DEVICE", 3: "OUT_OF_MEMORY", 4: "IO",
@@ -20,6 +20,14 @@ STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "DEVICE", 3: "OUT_OF_MEMORY", 4: "I
 class AtmoTables(ctypes.Structure):
     _fields_ = [("transmittance", ctypes.c_void_p), ("irradiance", ctypes.c_void_p),
                 ("inscatter", ctypes.c_void_p)]
+
+
+class AtmoPassTime(ctypes.Structure):
+    _fields_ = [("pass_", ctypes.c_int), ("order", ctypes.c_int), ("ms", ctypes.c_float)]
+
+
+PASS_NAMES = ("transmittance", "irradiance1", "inscatter1", "inscatterS", "irradianceN",
+              "inscatterN")
 
 
 class AtmoError(RuntimeError):
@@ -47,6 +55,8 @@ SIGNATURES = {
     "atmo_engine_destroy": (ctypes.c_int, [_vp]),
     "atmo_engine_precompute": (ctypes.c_int, [_vp, _pp, ctypes.c_int, _po,
                                               ctypes.POINTER(AtmoTables), _vp]),
+    "atmo_engine_pass_times": (ctypes.c_int, [_vp, ctypes.POINTER(AtmoPassTime), ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_int)]),
     "atmo_engine_workspace_size": (ctypes.c_int, [_pp, ctypes.c_int, _po,
                                                   ctypes.POINTER(ctypes.c_size_t)]),
     "atmo_pass_transmittance": (ctypes.c_int, [_vp, _pp, _po, _vp, _vp]),
@@ -155,6 +165,14 @@ class Engine:
         check(lib().atmo_engine_precompute(self.h, arr, len(sets), ctypes.byref(opts), tabs,
                                            _stream(stream)))
         return outs
+
+    def pass_times(self):
+        """[(pass name, order, ms)] of the last run made with FLAG_TIME_PASSES."""
+        n = ctypes.c_int()
+        check(lib().atmo_engine_pass_times(self.h, None, 0, ctypes.byref(n)))
+        buf = (AtmoPassTime * max(n.value, 1))()
+        check(lib().atmo_engine_pass_times(self.h, buf, n.value, ctypes.byref(n)))
+        return [(PASS_NAMES[buf[i].pass_], buf[i].order, buf[i].ms) for i in range(n.value)]
 
     # ---- pass level (float4 device tables) ----
     def pass_transmittance(self, p, opts, T4, stream=None):
