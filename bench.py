@@ -50,14 +50,14 @@ def time_steps(eng, sets, o, outs, steps, warmup):
 
 def cpu_baseline(p, o, budget_threads=16):
     """FP64 CPU oracle on a bounded sample of the same workload: the full-size
-    transmittance and irradiance passes plus the 4D passes at 8x32x16x8
-    (1/512 of the texels), extrapolated linearly in texel count."""
+    transmittance and irradiance passes plus the 4D passes at 8x64x16x8
+    (1/256 of the texels), extrapolated linearly in texel count."""
     import numpy as np
     from oracle import oracle as O
     from prolandatmo.params import options, with_res
     threads = min(budget_threads, os.cpu_count() or 1)
     O.set_threads(threads)
-    q = with_res(p, 8, 32, 16, 8)
+    q = with_res(p, 8, 64, 16, 8)
     scale = (p.RES_R * p.RES_MU * p.RES_MU_S * p.RES_NU) / float(q.RES_R * q.RES_MU * q.RES_MU_S * q.RES_NU)
     t = {}
     c = time.perf_counter
@@ -76,7 +76,7 @@ def cpu_baseline(p, o, budget_threads=16):
     est = tE + t4 * scale
     return {"value": 1.0 / est, "unit": "precomputes/s", "cores": threads, "kind": "port",
             "sample": ("FP64 oracle (oracle/, builder's CPU restatement; the reference GL preprocess "
-                       "cannot run here) on the same workload with the 4D table reduced to 8x32x16x8 "
+                       "cannot run here) on the same workload with the 4D table reduced to 8x64x16x8 "
                        "(1/%d of the texels), full-size T/E passes; %.1f s measured, %.0f s "
                        "extrapolated per precompute" % (int(scale), tE + t4, est))}
 
@@ -157,6 +157,20 @@ def main():
                                            "S_4096_texels": errS}}
         log("reference config: %.3f ms" % (sec1 * 1e3))
         del outs1
+        # batched plain method: 256 seeded atmospheres of mixed inscatter resolution
+        from prolandatmo.params import SEED_BATCH, draw_sets
+        sets, _ = draw_sets(SEED_BATCH, 256)
+        outsb = [L.alloc_outputs(q) for q in sets]
+        secb = time_steps(eng, sets, orr, outsb, 2, 1)
+        texels = sum(q.n_texels4d() for q in sets)
+        extra["batched_256_sets"] = {
+            "workload": "256 sets (seed 0x5EED0003), inscatter mixed from 16x64x16x4, "
+                        "32x128x32x8, 48x192x48x12; T 256x64, E 64x16, orders 4, one graph",
+            "tables_per_s": round(len(sets) / secb, 2), "s_per_batch": round(secb, 4),
+            "texels_4d": texels,
+            "algorithmic_tflops": round(sum(F.total(q, orr) for q in sets) / secb / 1e12, 2)}
+        log("batched 256 sets: %.3f s" % secb)
+        del outsb
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

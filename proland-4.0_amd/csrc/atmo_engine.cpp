@@ -50,6 +50,8 @@ atmo_status fail(atmo_status st, const char *fmt, ...) {
 
 enum Pass { P_T = 0, P_E1, P_S1, P_J, P_EN, P_SN, P_CHECK, P_COUNT };
 
+constexpr size_t kXcds = 8;  // MI355X accelerator complex dies
+
 atmo_options default_options() {
     atmo_options o;
     memset(&o, 0, sizeof o);
@@ -320,6 +322,19 @@ void build_tiles(const std::vector<atmo_params> &ps, const atmo_options &o,
                              [](const std::pair<double, uint2> &a, const std::pair<double, uint2> &b) {
                                  return a.first > b.first;
                              });
+        if (pass == P_SN) {
+            // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2);
+            // deal them so that XCD x runs a contiguous band of mu rows, whose rays
+            // gather from overlapping parts of the dJ table.
+            const size_t nt = v.size(), chunk = (nt + kXcds - 1) / kXcds;
+            std::vector<std::pair<double, uint2>> w;
+            w.reserve(nt);
+            for (size_t p = 0; p < chunk * kXcds; ++p) {
+                const size_t src = (p % kXcds) * chunk + p / kXcds;
+                if (src < nt) w.push_back(v[src]);
+            }
+            v.swap(w);
+        }
         for (auto &e : v) tiles.push_back(e.second);
         cnt[pass] = (int)tiles.size() - off[pass];
     }
