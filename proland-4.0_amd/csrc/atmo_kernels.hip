@@ -253,8 +253,8 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
     for (int k = threadIdx.x; k < s.SW; k += blockDim.x) {
         float4 a = s.dE4[k];
         float4 b = s.dE4[min(k + 1, s.SW - 1)];
-        epair[2 * k] = a;
-        epair[2 * k + 1] = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.f);
+        epair[2 * k] = make_float4(a.x, a.y, a.z, b.x - a.x);
+        epair[2 * k + 1] = make_float4(b.y - a.y, b.z - a.z, 0.f, 0.f);
     }
     // (3) per theta: ground visibility, ground-to-x transmittance (:129-138), and
     // the theta-uniform parts of the lookup positions
@@ -299,9 +299,9 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
         const int it = ek / NP, kk = ek % NP;
         float4 a = vtmp[((size_t)it * NV + kk) * NT + n];
         float4 b = vtmp[((size_t)it * NV + kk + 1) * NT + n];
-        pairs[((size_t)(it * NP + kk) * NT + n) * 2] = a;
-        pairs[((size_t)(it * NP + kk) * NT + n) * 2 + 1] =
-            make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.f);
+        // packed as (V.rgb, D.r), (D.g, D.b): one 16-byte and one 8-byte LDS read
+        pairs[((size_t)(it * NP + kk) * NT + n) * 2] = make_float4(a.x, a.y, a.z, b.x - a.x);
+        pairs[((size_t)(it * NP + kk) * NT + n) * 2 + 1] = make_float4(b.y - a.y, b.z - a.z, 0.f, 0.f);
     }
     __syncthreads();
 
@@ -370,14 +370,16 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
                     const unsigned kk = min(__float2uint_rz(pos[g]), NUu);
                     const float f = __builtin_amdgcn_fractf(pos[g]);
                     const float4 *pp = pt + kk * (NT * 2);
-                    const float4 a = pp[0], d = pp[1];
-                    lv[g][0] = fmaf(f, d.x, a.x);
-                    lv[g][1] = fmaf(f, d.y, a.y);
-                    lv[g][2] = fmaf(f, d.z, a.z);
+                    const float4 a = pp[0];
+                    const float2 d = *(const float2 *)(pp + 1);
+                    lv[g][0] = fmaf(f, a.w, a.x);
+                    lv[g][1] = fmaf(f, d.x, a.y);
+                    lv[g][2] = fmaf(f, d.y, a.z);
                     if (FIRST) {
-                        const float4 am = pp[2], dm = pp[3];
-                        const float m0 = fmaf(f, dm.x, am.x), m1 = fmaf(f, dm.y, am.y),
-                                    m2 = fmaf(f, dm.z, am.z);
+                        const float4 am = pp[2];
+                        const float2 dm = *(const float2 *)(pp + 3);
+                        const float m0 = fmaf(f, am.w, am.x), m1 = fmaf(f, dm.x, am.y),
+                                    m2 = fmaf(f, dm.y, am.z);
                         const float nu1 = fmaf(pos[g], iposA, nposBi);
                         const float q1 = fmaf(nu1, nu1, 1.0f);
                         const float y1 = __builtin_amdgcn_rsqf(fmaf(m2g, nu1, g2p1));
@@ -393,10 +395,11 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
                         const float xg = __builtin_amdgcn_fmed3f(fmaf(pos[g], g1.x, g1.y), 0.0f, SWm1);
                         const unsigned kg = __float2uint_rz(xg);
                         const float fg = __builtin_amdgcn_fractf(xg);
-                        const float4 ea = epair[2 * kg], ed = epair[2 * kg + 1];
-                        lv[g][0] = fmaf(g0.x, fmaf(fg, ed.x, ea.x), lv[g][0]);
-                        lv[g][1] = fmaf(g0.y, fmaf(fg, ed.y, ea.y), lv[g][1]);
-                        lv[g][2] = fmaf(g0.z, fmaf(fg, ed.z, ea.z), lv[g][2]);
+                        const float4 ea = epair[2 * kg];
+                        const float2 ed = *(const float2 *)(epair + 2 * kg + 1);
+                        lv[g][0] = fmaf(g0.x, fmaf(fg, ea.w, ea.x), lv[g][0]);
+                        lv[g][1] = fmaf(g0.y, fmaf(fg, ed.x, ea.y), lv[g][1]);
+                        lv[g][2] = fmaf(g0.z, fmaf(fg, ed.y, ea.z), lv[g][2]);
                     }
                 }
 #pragma unroll
