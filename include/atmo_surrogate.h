@@ -1,0 +1,106 @@
+/*
+ * atmo_surrogate.h — C ABI of the learned multiple-scattering surrogate.
+ *
+ * No counterpart in the reference (src/atmo/sources/proland/preprocess/atmo/
+ * computes every scattering order with its GL passes).  A small MLP predicts
+ * the contribution of scattering orders >= 2 to the inscatter and irradiance
+ * tables; single scattering and transmittance stay exact (the plain passes).
+ * The plain multi-order path is the judge and the fallback.
+ *
+ * Network: 11 inputs (4 texel coordinates in [-1, 1] + 7 normalised
+ * atmosphere features) -> 4 x 256 SiLU -> 4 outputs (log-scaled rgb of the
+ * orders >= 2 contribution, standardised; the 4th output is trained to 0
+ * because orders >= 2 add nothing to alpha).  The three 256x256 layers run in
+ * BF16 on the matrix cores with FP32 accumulation; master weights, Adam state,
+ * the first and last layers are FP32.
+ *
+ * Parameter vector layout (FP32): W1[11][256], b1[256], then for l = 2..4
+ * Wl[256][256] (row = input), bl[256], then W5[256][4], b5[4].
+ */
+#ifndef ATMO_SURROGATE_H_
+#define ATMO_SURROGATE_H_
+
+#include "atmo.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct atmo_surrogate atmo_surrogate;
+
+/* Creates a surrogate on the engine's device with seeded initial weights. */
+ATMO_API atmo_status atmo_surrogate_create(atmo_surrogate **out, int device, uint64_t seed);
+ATMO_API atmo_status atmo_surrogate_destroy(atmo_surrogate *s);
+
+/* Number of FP32 parameters (201476) and host copies of them. */
+ATMO_API size_t atmo_surrogate_param_count(void);
+ATMO_API atmo_status atmo_surrogate_get_params(atmo_surrogate *s, float *host, size_t n);
+ATMO_API atmo_status atmo_surrogate_set_params(atmo_surrogate *s, const float *host, size_t n);
+
+/* Output standardisation: target_c = (log(max(v_c, 0) + eps) - mu_c) / sd_c. */
+ATMO_API atmo_status atmo_surrogate_get_norm(atmo_surrogate *s, float mu[3], float sd[3],
+                                             float *eps);
+ATMO_API atmo_status atmo_surrogate_set_norm(atmo_surrogate *s, const float mu[3],
+                                             const float sd[3], float eps);
+
+/* Training data from the plain method: each set is precomputed with `opts`
+ * (orders >= 2) and with 1 order; `samples_per_set` seeded inscatter texels
+ * and every irradiance texel of each set become rows.  Replaces any earlier
+ * dataset; sets the output standardisation from the data. */
+ATMO_API atmo_status atmo_surrogate_build_dataset(atmo_surrogate *s, const atmo_params *sets,
+                                                  int nsets, const atmo_options *opts,
+                                                  int samples_per_set, uint64_t seed,
+                                                  size_t *rows);
+
+/* Adam steps on random batches of `batch` rows (multiple of 128) with a
+ * cosine learning-rate schedule from lr to lr_end; losses[steps] (host, may be
+ * NULL) receives the mean-square loss of every step. */
+ATMO_API atmo_status atmo_surrogate_train(atmo_surrogate *s, int steps, int batch, float lr,
+                                          float lr_end, uint64_t seed, float *losses);
+
+/* One forward + backward pass on caller rows (device pointers X[B][16] with
+ * 11 used columns, T[B][4] targets, B a multiple of 128): mean-square loss
+ * and the gradient of every parameter (host, param_count floats). */
+ATMO_API atmo_status atmo_surrogate_loss_grad(atmo_surrogate *s, const float *X, const float *T,
+                                              int B, float *loss, float *grad_host);
+
+/* Network outputs for device rows X[B][16] into device Y[B][4] (any B > 0). */
+ATMO_API atmo_status atmo_surrogate_predict(atmo_surrogate *s, const float *X, int B, float *Y);
+
+/* Features of texels of one set (codes < n4: inscatter texel index; codes with
+ * bit 31 set: irradiance texel index) into device X[n][16]. */
+ATMO_API atmo_status atmo_surrogate_features(const atmo_params *p, const uint32_t *codes, int n,
+                                             float *X);
+
+typedef struct atmo_surrogate_report {
+    int nsets;            /* sets in the batch */
+    int judged;           /* sets recomputed by the plain method as judge */
+    int accepted;         /* 1 if every judged set met the tolerance */
+    int fallback_sets;    /* sets whose tables come from the plain method */
+    float max_rel_err_S;  /* worst judged inscatter error (floored at 1e-4 of the table max) */
+    float max_rel_err_E;  /* worst judged irradiance error */
+    float ms_exact;       /* exact transmittance + single scattering */
+    float ms_mlp;         /* network inference + combination */
+    float ms_judge;       /* plain recompute of the judged sets + comparison */
+    float ms_fallback;    /* plain recompute of the rejected batch */
+} atmo_surrogate_report;
+
+/* Surrogate-accelerated precompute of nsets sets into caller device tables:
+ * exact transmittance and single scattering, network-predicted orders >= 2,
+ * then a judge: ceil(judge_fraction * nsets) seeded sets are recomputed with
+ * the plain multi-order path and compared.  If every judged set is within
+ * `tolerance` (max relative error, floored at 1e-4 of the table maximum) the
+ * batch is accepted (judged sets keep their plain tables); otherwise every set
+ * falls back to the plain path.  Plain results are bit-identical to
+ * atmo_engine_precompute. */
+ATMO_API atmo_status atmo_surrogate_precompute(atmo_surrogate *s, const atmo_params *sets,
+                                               int nsets, const atmo_options *opts,
+                                               const atmo_tables *out, float judge_fraction,
+                                               float tolerance, uint64_t seed,
+                                               atmo_surrogate_report *report, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -211,6 +211,14 @@ struct Plan {
 
 }  // namespace
 
+namespace atmo {
+// Sets the calling thread's atmo_last_error() message (used by the surrogate).
+atmo_status set_error(atmo_status st, const char *msg) {
+    g_err = msg;
+    return st;
+}
+}  // namespace atmo
+
 struct atmo_engine {
     int device = 0;
     hipStream_t cap = nullptr;  // capture stream

@@ -1,0 +1,63 @@
+// surrogate_kernels.h — launchers of the surrogate MLP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace atmo {
+namespace sg {
+
+constexpr int kNIN = 11;   // 4 texel coordinates + 7 atmosphere features
+constexpr int kXS = 16;    // row stride of the input features (floats)
+constexpr int kNH = 256;   // hidden width
+constexpr int kNL = 4;     // hidden layers
+constexpr int kNOUT = 4;   // outputs (rgb + an always-zero alpha)
+
+enum { EPI_FWD = 0, EPI_FWD_INF = 1, EPI_DX = 2, EPI_PARTIAL = 3 };
+
+struct GemmEpi {
+    const float *bias;   // EPI_FWD*: bias[N]
+    const float *zin;    // EPI_DX: pre-activations of the layer below
+    float *zout;         // EPI_FWD: pre-activations; EPI_DX: optional FP32 copy; EPI_PARTIAL: partials
+    __bf16 *hout;        // EPI_FWD*: activations; EPI_DX: gradients
+};
+
+struct FeatSet {
+    int RR, RMU, RMUS, RNU, SW, SH, n4, pad;
+    float f[7];          // normalised atmosphere features
+};
+
+struct TargetNorm {
+    float mu[3], sd[3], eps;
+};
+
+hipError_t gemm(bool ta, bool tb, int epi, const __bf16 *A, const __bf16 *B, int M, int N, int K,
+                int nsplit, const GemmEpi &ep, hipStream_t st);
+hipError_t layer1_fwd(const float *X, const float *W1, const float *b1, int rows, float *Z1,
+                      __bf16 *H1, hipStream_t st);
+hipError_t layer5(const __bf16 *H4, const float *W5, const float *b5, const float *Tg, int rows,
+                  int mode, float *Y, float *dY, float *loss, hipStream_t st);
+hipError_t layer5_bwd(const __bf16 *H4, const float *Z4, const float *W5, const float *dY, int rows,
+                      float *gW5, float *gb5, __bf16 *dZ4, hipStream_t st);
+hipError_t layer1_bwd(const float *X, const float *dZ1, int rows, float *gW1, float *gb1,
+                      hipStream_t st);
+hipError_t colsum(const __bf16 *D, int rows, float *g, hipStream_t st);
+hipError_t reduce_partials(const float *P, int nsplit, int n, float *g, hipStream_t st);
+hipError_t adam(float *p, const float *g, float *m, float *v, int n, float lr, float b1, float b2,
+                float eps, float c1, float c2, hipStream_t st);
+hipError_t to_bf16(const float *W, __bf16 *Wb, __bf16 *Wt, hipStream_t st);
+hipError_t make_rows(const FeatSet &fs, const float4 *S4, const float4 *S1, const float *E, int ns,
+                     unsigned long long seed, float *X, float *Tg, const TargetNorm &tn,
+                     hipStream_t st);
+hipError_t infer_rows(const FeatSet &fs, int row0, int n, float *X, hipStream_t st);
+hipError_t code_rows(const FeatSet &fs, const unsigned *codes, int n, float *X, hipStream_t st);
+hipError_t norm_targets(float *Tg, size_t n, const TargetNorm &tn, hipStream_t st);
+hipError_t apply(int row0, int n, int n4, const float *Y, const TargetNorm &tn, float4 *S, float *E,
+                 hipStream_t st);
+hipError_t gather(const float *Xa, const float *Ta, size_t n, int B, unsigned long long seed,
+                  float *X, float *Tg, hipStream_t st);
+hipError_t relerr(const float *a, const float *b, size_t n, int stride, int ncomp, float floorv,
+                  unsigned *out, hipStream_t st);
+hipError_t absmax(const float *a, size_t n, int stride, int ncomp, unsigned *out, hipStream_t st);
+
+}  // namespace sg
+}  // namespace atmo

@@ -1,0 +1,564 @@
+// surrogate_kernels.hip — kernels of the learned multiple-scattering surrogate.
+//
+// MLP 11 -> 256 -> 256 -> 256 -> 256 -> 4, SiLU on the hidden layers.  The
+// three 256x256 layers (97 % of the FLOPs) run on the BF16 matrix cores
+// (v_mfma_f32_32x32x16_bf16, FP32 accumulation) in one LDS-tiled GEMM kernel
+// with transposable operands and fused epilogues; the thin first and last
+// layers, the loss, the reductions and Adam are FP32 vector code.
+#include "surrogate_kernels.h"
+
+namespace atmo {
+namespace sg {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
+__device__ __forceinline__ float dsilu(float z) {
+    const float s = sigm(z);
+    return s * (1.0f + z * (1.0f - s));
+}
+
+// ---------------------------------------------------------------------------
+// C[M x N] = op(A)[M x K] . op(B)[K x N], bf16 in, FP32 accumulate.
+//   TA = false: A stored [M][K] (lda = K);  TA = true: A stored [K][M] (lda = M)
+//   TB = true : B stored [N][K] (ldb = K);  TB = false: B stored [K][N] (ldb = N)
+// Tile 128 x 128 x 32, 4 waves of 64 x 64 (2 x 2 MFMA 32x32 tiles).  blockIdx.z
+// splits K into chunks of kchunk (EPI_PARTIAL writes one partial per chunk).
+// M, N multiples of 128, K and kchunk multiples of 32 (checked by the host).
+// ---------------------------------------------------------------------------
+template <bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A,
+                                              const __bf16 *__restrict__ Bm, int M, int N, int K,
+                                              int kchunk, GemmEpi ep) {
+    __shared__ __attribute__((aligned(16))) __bf16 As[128][40];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[128][40];
+    const int n0 = blockIdx.x * 128, m0 = blockIdx.y * 128;
+    const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wm = w >> 1, wn = w & 1;
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    for (int k0 = kb; k0 < ke; k0 += 32) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int c = tid + i * 256;
+            if (!TA) {
+                const int row = c >> 2, kc = (c & 3) * 8;
+                const uint4 v = *(const uint4 *)(A + (size_t)(m0 + row) * K + k0 + kc);
+                *(uint4 *)&As[row][kc] = v;
+            } else {
+                const int kr = c >> 4, mc = (c & 15) * 8;
+                const uint4 v = *(const uint4 *)(A + (size_t)(k0 + kr) * M + m0 + mc);
+                const __bf16 *pv = (const __bf16 *)&v;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) As[mc + j][kr] = pv[j];
+            }
+            if (TB) {
+                const int n = c >> 2, kc = (c & 3) * 8;
+                const uint4 v = *(const uint4 *)(Bm + (size_t)(n0 + n) * K + k0 + kc);
+                *(uint4 *)&Bs[n][kc] = v;
+            } else {
+                const int kr = c >> 4, nc = (c & 15) * 8;
+                const uint4 v = *(const uint4 *)(Bm + (size_t)(k0 + kr) * N + n0 + nc);
+                const __bf16 *pv = (const __bf16 *)&v;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) Bs[nc + j][kr] = pv[j];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 32; ks += 16) {
+            bf16x8 a[2], b[2];
+            const int kk = ks + 8 * (lane >> 5);
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                a[t] = *(const bf16x8 *)&As[wm * 64 + t * 32 + (lane & 31)][kk];
+                b[t] = *(const bf16x8 *)&Bs[wn * 64 + t * 32 + (lane & 31)][kk];
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    // epilogue: lane holds rows 8(r/4) + 4(lane/32) + r%4 of column lane%32 of each 32x32 tile
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int col = n0 + wn * 64 + j * 32 + (lane & 31);
+                const size_t o = (size_t)row * N + col;
+                const float v = acc[i][j][r];
+                if (EPI == EPI_FWD) {
+                    const float z = v + ep.bias[col];
+                    ep.zout[o] = z;
+                    ep.hout[o] = (__bf16)silu(z);
+                } else if (EPI == EPI_FWD_INF) {
+                    ep.hout[o] = (__bf16)silu(v + ep.bias[col]);
+                } else if (EPI == EPI_DX) {
+                    const float g = v * dsilu(ep.zin[o]);
+                    ep.hout[o] = (__bf16)g;
+                    if (ep.zout) ep.zout[o] = g;
+                } else {  // EPI_PARTIAL
+                    ep.zout[(size_t)blockIdx.z * M * N + o] = v;
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
+// Layer 1 (11 -> 256), FP32: Z1 = X W1 + b1, H1 = SiLU(Z1).  X rows are 16
+// floats (11 used).  One thread per (row, column).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_layer1_fwd(const float *__restrict__ X,
+                                                    const float *__restrict__ W1,
+                                                    const float *__restrict__ b1, int rows,
+                                                    float *__restrict__ Z1, __bf16 *__restrict__ H1) {
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (size_t)rows * kNH) return;
+    const int n = (int)(t % kNH);
+    const size_t b = t / kNH;
+    float z = b1[n];
+#pragma unroll
+    for (int i = 0; i < kNIN; ++i) z = fmaf(X[b * kXS + i], W1[i * kNH + n], z);
+    if (Z1) Z1[t] = z;
+    H1[t] = (__bf16)silu(z);
+}
+
+// ---------------------------------------------------------------------------
+// Layer 5 (256 -> 4), FP32 from bf16 H4, fused with the loss: one wave per
+// row, each lane 4 of the 256 inputs, shuffles for the dot products.
+//   mode 0 (train):   dY = 2 (Y - T) / (rows * 4), loss += sum (Y - T)^2 / (rows * 4)
+//   mode 1 (predict): Y written
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_layer5(const __bf16 *__restrict__ H4,
+                                                const float *__restrict__ W5,
+                                                const float *__restrict__ b5,
+                                                const float *__restrict__ Tg, int rows, int mode,
+                                                float *__restrict__ Y, float *__restrict__ dY,
+                                                float *__restrict__ loss) {
+    const int lane = threadIdx.x & 63;
+    const size_t b = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= (size_t)rows) return;
+    float acc[kNOUT] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int k = lane * 4 + q;
+        const float h = (float)H4[b * kNH + k];
+#pragma unroll
+        for (int o = 0; o < kNOUT; ++o) acc[o] = fmaf(h, W5[k * kNOUT + o], acc[o]);
+    }
+#pragma unroll
+    for (int o = 0; o < kNOUT; ++o)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc[o] += __shfl_xor(acc[o], off, 64);
+    if (lane == 0) {
+        const float inv = 1.0f / ((float)rows * kNOUT);
+        float l = 0.f;
+#pragma unroll
+        for (int o = 0; o < kNOUT; ++o) {
+            const float y = acc[o] + b5[o];
+            if (mode == 0) {
+                const float d = y - Tg[b * kNOUT + o];
+                dY[b * kNOUT + o] = 2.0f * d * inv;
+                l += d * d;
+            } else {
+                Y[b * kNOUT + o] = y;
+            }
+        }
+        if (mode == 0) atomicAdd(loss, l * inv);
+    }
+}
+
+// Layer 5 backward: dW5 += H4^T dY, db5 += sum dY, dZ4 = (dY W5^T) * SiLU'(Z4).
+// One wave per row as in the forward.
+__global__ __launch_bounds__(256) void k_layer5_bwd(const __bf16 *__restrict__ H4,
+                                                    const float *__restrict__ Z4,
+                                                    const float *__restrict__ W5,
+                                                    const float *__restrict__ dY, int rows,
+                                                    float *__restrict__ gW5, float *__restrict__ gb5,
+                                                    __bf16 *__restrict__ dZ4) {
+    __shared__ float part[4][kNH * kNOUT];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < 4 * kNH * kNOUT; i += 256) (&part[0][0])[i] = 0.f;
+    __syncthreads();
+    // each block handles 64 rows (16 per wave)
+    float dsum = 0.f;  // lane o < 4: sum of dY[., o] over this wave's rows
+    for (int rr = 0; rr < 16; ++rr) {
+        const size_t b = (size_t)blockIdx.x * 64 + wv * 16 + rr;
+        if (b >= (size_t)rows) break;
+        float d[kNOUT];
+#pragma unroll
+        for (int o = 0; o < kNOUT; ++o) d[o] = dY[b * kNOUT + o];
+        dsum += lane == 0 ? d[0] : lane == 1 ? d[1] : lane == 2 ? d[2] : lane == 3 ? d[3] : 0.f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = lane * 4 + q;
+            const size_t idx = b * kNH + k;
+            const float h = (float)H4[idx];
+            float g = 0.f;
+#pragma unroll
+            for (int o = 0; o < kNOUT; ++o) {
+                part[wv][k * kNOUT + o] += h * d[o];
+                g = fmaf(d[o], W5[k * kNOUT + o], g);
+            }
+            dZ4[idx] = (__bf16)(g * dsilu(Z4[idx]));
+        }
+    }
+    if (lane < kNOUT) atomicAdd(&gb5[lane], dsum);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kNH * kNOUT; i += 256)
+        atomicAdd(&gW5[i], part[0][i] + part[1][i] + part[2][i] + part[3][i]);
+}
+
+// Layer 1 backward: dW1 += X^T dZ1 (11 x 256), db1 += sum dZ1.  Block = 256
+// columns x a chunk of rows.
+__global__ __launch_bounds__(256) void k_layer1_bwd(const float *__restrict__ X,
+                                                    const float *__restrict__ dZ1, int rows,
+                                                    int chunk, float *__restrict__ gW1,
+                                                    float *__restrict__ gb1) {
+    const int n = threadIdx.x;
+    const size_t r0 = (size_t)blockIdx.x * chunk;
+    const size_t r1 = min((size_t)rows, r0 + chunk);
+    float a[kNIN], s = 0.f;
+#pragma unroll
+    for (int i = 0; i < kNIN; ++i) a[i] = 0.f;
+    for (size_t b = r0; b < r1; ++b) {
+        const float g = dZ1[b * kNH + n];
+        s += g;
+#pragma unroll
+        for (int i = 0; i < kNIN; ++i) a[i] = fmaf(X[b * kXS + i], g, a[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < kNIN; ++i) atomicAdd(&gW1[i * kNH + n], a[i]);
+    atomicAdd(&gb1[n], s);
+}
+
+// Column sums of a bf16 [rows x 256] matrix into g (bias gradients).
+__global__ __launch_bounds__(256) void k_colsum(const __bf16 *__restrict__ D, int rows, int chunk,
+                                                float *__restrict__ g) {
+    const int n = threadIdx.x;
+    const size_t r0 = (size_t)blockIdx.x * chunk;
+    const size_t r1 = min((size_t)rows, r0 + chunk);
+    float s = 0.f;
+    for (size_t b = r0; b < r1; ++b) s += (float)D[b * kNH + n];
+    atomicAdd(&g[n], s);
+}
+
+// g[i] += sum over nsplit partials P[s][i].
+__global__ __launch_bounds__(256) void k_reduce_partials(const float *__restrict__ P, int nsplit,
+                                                         int n, float *__restrict__ g) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int k = 0; k < nsplit; ++k) s += P[(size_t)k * n + i];
+    g[i] += s;
+}
+
+// Adam (bias-corrected) on the flat FP32 parameter vector.
+__global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, const float *__restrict__ g,
+                                              float *__restrict__ m, float *__restrict__ v, int n,
+                                              float lr, float b1, float b2, float eps, float c1,
+                                              float c2) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+// bf16 copies of a 256x256 weight: W (row-major [in][out]) and Wt ([out][in]).
+__global__ __launch_bounds__(256) void k_to_bf16(const float *__restrict__ W, __bf16 *__restrict__ Wb,
+                                                 __bf16 *__restrict__ Wt) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= kNH * kNH) return;
+    const int r = i / kNH, c = i % kNH;
+    const __bf16 v = (__bf16)W[i];
+    Wb[i] = v;
+    Wt[c * kNH + r] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Features of table texels and dataset assembly
+// ---------------------------------------------------------------------------
+// Resolution-independent texel coordinates in [-1, 1] (the arguments of
+// getMuMuSNu, common.glsl:175-201) plus the 7 normalised atmosphere features.
+// E texels use the irradiance coordinates in slots 0 and 2 and -2 in 1 and 3.
+__device__ __forceinline__ void texel_features(const FeatSet &fs, unsigned code, float *x) {
+    if (code & 0x80000000u) {
+        const unsigned t = code & 0x7fffffffu;
+        const int i = t % fs.SW, j = t / fs.SW;
+        x[0] = 2.f * j / (fs.SH - 1) - 1.f;
+        x[1] = -2.f;
+        x[2] = 2.f * i / (fs.SW - 1) - 1.f;
+        x[3] = -2.f;
+    } else {
+        const int W4 = fs.RMUS * fs.RNU;
+        const int xx = code % W4, y = (code / W4) % fs.RMU, l = code / (W4 * fs.RMU);
+        const int half = fs.RMU / 2;
+        const float umu = y < half ? 0.5f * y / (half - 1) : 0.5f + 0.5f * (y - half) / (half - 1);
+        x[0] = 2.f * l / (fs.RR - 1) - 1.f;
+        x[1] = 2.f * umu - 1.f;
+        x[2] = 2.f * (xx % fs.RMUS) / (fs.RMUS - 1) - 1.f;
+        x[3] = 2.f * (xx / fs.RMUS) / (fs.RNU - 1) - 1.f;
+    }
+#pragma unroll
+    for (int k = 0; k < 7; ++k) x[4 + k] = fs.f[k];
+}
+
+__device__ __forceinline__ unsigned hash32(unsigned long long z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return (unsigned)((z ^ (z >> 31)) >> 32);
+}
+
+// Training rows of one set: `ns` seeded S texels then all E texels.
+// target c = (log(max(v_c, 0) + eps) - mu_c) / sd_c, v = S(orders<=N) - S(order 1) for S
+// texels (rgb) and E for E texels; the fourth target is 0 (orders >= 2 add 0 to alpha).
+__global__ __launch_bounds__(256) void k_make_rows(FeatSet fs, const float4 *__restrict__ S4,
+                                                   const float4 *__restrict__ S1,
+                                                   const float *__restrict__ E, int ns,
+                                                   unsigned long long seed, float *__restrict__ X,
+                                                   float *__restrict__ Tg, TargetNorm tn) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int nE = fs.SW * fs.SH;
+    if (t >= ns + nE) return;
+    unsigned code;
+    float v[3];
+    if (t < ns) {
+        code = hash32(seed * 0x100000001ull + t) % (unsigned)fs.n4;
+        const float4 a = S4[code], b = S1[code];
+        v[0] = a.x - b.x; v[1] = a.y - b.y; v[2] = a.z - b.z;
+    } else {
+        const int e = t - ns;
+        code = 0x80000000u | (unsigned)e;
+        v[0] = E[3 * e]; v[1] = E[3 * e + 1]; v[2] = E[3 * e + 2];
+    }
+    float x[kNIN];
+    texel_features(fs, code, x);
+#pragma unroll
+    for (int i = 0; i < kNIN; ++i) X[(size_t)t * kXS + i] = x[i];
+#pragma unroll
+    for (int i = kNIN; i < kXS; ++i) X[(size_t)t * kXS + i] = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Tg[(size_t)t * kNOUT + c] = (logf(fmaxf(v[c], 0.f) + tn.eps) - tn.mu[c]) / tn.sd[c];
+    Tg[(size_t)t * kNOUT + 3] = 0.f;
+}
+
+// Inference rows of one set: rows [row0, row0 + n) of the set's S texels
+// (code < n4) and then its E texels.
+__global__ __launch_bounds__(256) void k_infer_rows(FeatSet fs, int row0, int n,
+                                                    float *__restrict__ X) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const int r = row0 + t;
+    const unsigned code = r < fs.n4 ? (unsigned)r : (0x80000000u | (unsigned)(r - fs.n4));
+    float x[kNIN];
+    texel_features(fs, code, x);
+#pragma unroll
+    for (int i = 0; i < kNIN; ++i) X[(size_t)t * kXS + i] = x[i];
+#pragma unroll
+    for (int i = kNIN; i < kXS; ++i) X[(size_t)t * kXS + i] = 0.f;
+}
+
+// Rows for caller-given texel codes.
+__global__ __launch_bounds__(256) void k_code_rows(FeatSet fs, const unsigned *__restrict__ codes,
+                                                   int n, float *__restrict__ X) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    float x[kNIN];
+    texel_features(fs, codes[t], x);
+#pragma unroll
+    for (int i = 0; i < kNIN; ++i) X[(size_t)t * kXS + i] = x[i];
+#pragma unroll
+    for (int i = kNIN; i < kXS; ++i) X[(size_t)t * kXS + i] = 0.f;
+}
+
+// In-place standardisation of the three rgb targets.
+__global__ __launch_bounds__(256) void k_norm_targets(float *__restrict__ Tg, size_t n, TargetNorm tn) {
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Tg[t * kNOUT + c] = (Tg[t * kNOUT + c] - tn.mu[c]) / tn.sd[c];
+}
+
+// Adds the predicted orders >= 2 to the exact single scattering (S.rgb) and
+// sets E.  Y rows follow k_infer_rows.
+__global__ __launch_bounds__(256) void k_apply(int row0, int n, int n4, const float *__restrict__ Y,
+                                               TargetNorm tn, float4 *__restrict__ S,
+                                               float *__restrict__ E) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const int r = row0 + t;
+    float v[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = fmaxf(__expf(Y[(size_t)t * kNOUT + c] * tn.sd[c] + tn.mu[c]) - tn.eps, 0.f);
+    if (r < n4) {
+        float4 s = S[r];
+        s.x += v[0]; s.y += v[1]; s.z += v[2];
+        S[r] = s;
+    } else {
+        const int e = r - n4;
+        E[3 * e] = v[0]; E[3 * e + 1] = v[1]; E[3 * e + 2] = v[2];
+    }
+}
+
+// Random training batch: rows idx = hash(seed, b) mod n copied from the dataset.
+__global__ __launch_bounds__(256) void k_gather(const float *__restrict__ Xa, const float *__restrict__ Ta,
+                                                size_t n, int B, unsigned long long seed,
+                                                float *__restrict__ X, float *__restrict__ Tg) {
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= B) return;
+    const unsigned long long h = ((unsigned long long)hash32(seed * 0x9E3779B1ull + 2 * b) << 32) |
+                                 hash32(seed * 0x9E3779B1ull + 2 * b + 1);
+    const size_t i = (size_t)(h % n);
+#pragma unroll
+    for (int k = 0; k < kXS; ++k) X[(size_t)b * kXS + k] = Xa[i * kXS + k];
+#pragma unroll
+    for (int k = 0; k < kNOUT; ++k) Tg[(size_t)b * kNOUT + k] = Ta[i * kNOUT + k];
+}
+
+// Per-set max relative error of a surrogate table against the plain one,
+// floored at 1e-4 of the plain table's largest magnitude (scale given).
+__global__ __launch_bounds__(256) void k_relerr(const float *__restrict__ a, const float *__restrict__ b,
+                                                size_t n, int stride, int ncomp, float floorv,
+                                                unsigned int *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    float e = 0.f;
+    if (i < n) {
+        for (int c = 0; c < ncomp; ++c) {
+            const float x = a[i * stride + c], y = b[i * stride + c];
+            const float den = fmaxf(fabsf(y), floorv);
+            float r = fabsf(x - y) / den;
+            if (!(r == r)) r = 3.0e38f;  // NaN counts as a failure
+            e = fmaxf(e, r);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) e = fmaxf(e, __shfl_xor(e, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(e));
+}
+
+// max |v| over a table (for the relative-error floor)
+__global__ __launch_bounds__(256) void k_absmax(const float *__restrict__ a, size_t n, int stride,
+                                                int ncomp, unsigned int *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    float e = 0.f;
+    if (i < n)
+        for (int c = 0; c < ncomp; ++c) e = fmaxf(e, fabsf(a[i * stride + c]));
+    for (int off = 32; off > 0; off >>= 1) e = fmaxf(e, __shfl_xor(e, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(e));
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t gemm(bool ta, bool tb, int epi, const __bf16 *A, const __bf16 *B, int M, int N, int K,
+                int nsplit, const GemmEpi &ep, hipStream_t st) {
+    if (M % 128 || N % 128 || K % 32 || nsplit < 1 || (K / nsplit) % 32) return hipErrorInvalidValue;
+    dim3 grid(N / 128, M / 128, nsplit);
+    const int kc = K / nsplit;
+#define SG_L(TA_, TB_, E_) hipLaunchKernelGGL((k_gemm<TA_, TB_, E_>), grid, dim3(256), 0, st, A, B, M, N, K, kc, ep)
+    if (!ta && tb && epi == EPI_FWD) SG_L(false, true, EPI_FWD);
+    else if (!ta && tb && epi == EPI_FWD_INF) SG_L(false, true, EPI_FWD_INF);
+    else if (!ta && tb && epi == EPI_DX) SG_L(false, true, EPI_DX);
+    else if (ta && !tb && epi == EPI_PARTIAL) SG_L(true, false, EPI_PARTIAL);
+    else return hipErrorInvalidValue;
+#undef SG_L
+    return hipGetLastError();
+}
+
+static unsigned blocks(size_t n) { return (unsigned)((n + 255) / 256); }
+
+hipError_t layer1_fwd(const float *X, const float *W1, const float *b1, int rows, float *Z1, __bf16 *H1,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(k_layer1_fwd, dim3(blocks((size_t)rows * kNH)), dim3(256), 0, st, X, W1, b1, rows, Z1, H1);
+    return hipGetLastError();
+}
+hipError_t layer5(const __bf16 *H4, const float *W5, const float *b5, const float *Tg, int rows, int mode,
+                  float *Y, float *dY, float *loss, hipStream_t st) {
+    hipLaunchKernelGGL(k_layer5, dim3((rows + 3) / 4), dim3(256), 0, st, H4, W5, b5, Tg, rows, mode, Y, dY, loss);
+    return hipGetLastError();
+}
+hipError_t layer5_bwd(const __bf16 *H4, const float *Z4, const float *W5, const float *dY, int rows,
+                      float *gW5, float *gb5, __bf16 *dZ4, hipStream_t st) {
+    hipLaunchKernelGGL(k_layer5_bwd, dim3((rows + 63) / 64), dim3(256), 0, st, H4, Z4, W5, dY, rows, gW5, gb5, dZ4);
+    return hipGetLastError();
+}
+hipError_t layer1_bwd(const float *X, const float *dZ1, int rows, float *gW1, float *gb1, hipStream_t st) {
+    const int chunk = 256;
+    hipLaunchKernelGGL(k_layer1_bwd, dim3((rows + chunk - 1) / chunk), dim3(256), 0, st, X, dZ1, rows, chunk, gW1, gb1);
+    return hipGetLastError();
+}
+hipError_t colsum(const __bf16 *D, int rows, float *g, hipStream_t st) {
+    const int chunk = 256;
+    hipLaunchKernelGGL(k_colsum, dim3((rows + chunk - 1) / chunk), dim3(256), 0, st, D, rows, chunk, g);
+    return hipGetLastError();
+}
+hipError_t reduce_partials(const float *P, int nsplit, int n, float *g, hipStream_t st) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(blocks(n)), dim3(256), 0, st, P, nsplit, n, g);
+    return hipGetLastError();
+}
+hipError_t adam(float *p, const float *g, float *m, float *v, int n, float lr, float b1, float b2,
+                float eps, float c1, float c2, hipStream_t st) {
+    hipLaunchKernelGGL(k_adam, dim3(blocks(n)), dim3(256), 0, st, p, g, m, v, n, lr, b1, b2, eps, c1, c2);
+    return hipGetLastError();
+}
+hipError_t to_bf16(const float *W, __bf16 *Wb, __bf16 *Wt, hipStream_t st) {
+    hipLaunchKernelGGL(k_to_bf16, dim3(blocks(kNH * kNH)), dim3(256), 0, st, W, Wb, Wt);
+    return hipGetLastError();
+}
+hipError_t make_rows(const FeatSet &fs, const float4 *S4, const float4 *S1, const float *E, int ns,
+                     unsigned long long seed, float *X, float *Tg, const TargetNorm &tn, hipStream_t st) {
+    const int n = ns + fs.SW * fs.SH;
+    hipLaunchKernelGGL(k_make_rows, dim3(blocks(n)), dim3(256), 0, st, fs, S4, S1, E, ns, seed, X, Tg, tn);
+    return hipGetLastError();
+}
+hipError_t infer_rows(const FeatSet &fs, int row0, int n, float *X, hipStream_t st) {
+    hipLaunchKernelGGL(k_infer_rows, dim3(blocks(n)), dim3(256), 0, st, fs, row0, n, X);
+    return hipGetLastError();
+}
+hipError_t code_rows(const FeatSet &fs, const unsigned *codes, int n, float *X, hipStream_t st) {
+    hipLaunchKernelGGL(k_code_rows, dim3(blocks(n)), dim3(256), 0, st, fs, codes, n, X);
+    return hipGetLastError();
+}
+hipError_t norm_targets(float *Tg, size_t n, const TargetNorm &tn, hipStream_t st) {
+    hipLaunchKernelGGL(k_norm_targets, dim3(blocks(n)), dim3(256), 0, st, Tg, n, tn);
+    return hipGetLastError();
+}
+hipError_t apply(int row0, int n, int n4, const float *Y, const TargetNorm &tn, float4 *S, float *E,
+                 hipStream_t st) {
+    hipLaunchKernelGGL(k_apply, dim3(blocks(n)), dim3(256), 0, st, row0, n, n4, Y, tn, S, E);
+    return hipGetLastError();
+}
+hipError_t gather(const float *Xa, const float *Ta, size_t n, int B, unsigned long long seed, float *X,
+                  float *Tg, hipStream_t st) {
+    hipLaunchKernelGGL(k_gather, dim3(blocks(B)), dim3(256), 0, st, Xa, Ta, n, B, seed, X, Tg);
+    return hipGetLastError();
+}
+hipError_t relerr(const float *a, const float *b, size_t n, int stride, int ncomp, float floorv,
+                  unsigned *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_relerr, dim3(blocks(n)), dim3(256), 0, st, a, b, n, stride, ncomp, floorv, out);
+    return hipGetLastError();
+}
+hipError_t absmax(const float *a, size_t n, int stride, int ncomp, unsigned *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_absmax, dim3(blocks(n)), dim3(256), 0, st, a, n, stride, ncomp, out);
+    return hipGetLastError();
+}
+
+}  // namespace sg
+}  // namespace atmo

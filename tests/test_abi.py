@@ -19,12 +19,18 @@ def declared(header):
 
 
 def test_library_exports_every_declared_symbol():
+    from prolandatmo import surrogate as SG
     names = declared("atmo.h")
     assert len(names) >= 20
     lib = L.lib()
     for n in names:
         assert hasattr(lib, n), n
     assert names == set(L.SIGNATURES), "binding and header disagree"
+    snames = declared("atmo_surrogate.h")
+    assert snames == set(SG.SIGNATURES)
+    for n in snames:
+        assert hasattr(lib, n), n
+    assert SG.param_count() == 201476
     syms = os.popen("nm -D --defined-only %s" % L.LIB_PATH).read()
     for mangled in ("_ZN7proland14preprocessAtmoERKNS_14AtmoParametersEPKc",
                     "_ZN7proland14AtmoParametersC1Ev", "_ZN7proland20preprocessAtmoStatusEv"):

@@ -1,0 +1,136 @@
+"""GPU tests of the learned multiple-scattering surrogate: BF16-MFMA forward
+and backward against the NumPy MLP oracle, training, and the judge/fallback
+contract of atmo_surrogate_precompute."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import mlp_oracle as M  # noqa: E402
+from prolandatmo import lib as L  # noqa: E402
+from prolandatmo import surrogate as SG  # noqa: E402
+from prolandatmo.params import SEED_BATCH, draw_sets, options  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sur():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    s = SG.Surrogate(0, seed=7)
+    yield s
+    s.close()
+
+
+def rows(B, seed):
+    rng = np.random.default_rng(seed)
+    X = np.zeros((B, 16), dtype=np.float32)
+    X[:, :11] = rng.uniform(-1, 1, (B, 11))
+    T = rng.normal(0, 1, (B, 4)).astype(np.float32)
+    return X, T
+
+
+def blocks(v):
+    lay = SG.layout()
+    return {k: v[o:o + int(np.prod(s))] for k, (o, s) in lay.items()}
+
+
+def test_forward_matches_oracle(sur):
+    X, _ = rows(300, 1)
+    Y = sur.predict(torch.from_numpy(X).cuda()).double().cpu().numpy()
+    Yo, _ = M.forward(sur.get_params().astype(np.float64), X, emulate_bf16=True)
+    err = np.abs(Y - Yo).max() / np.abs(Yo).max()
+    print("forward max rel err %.3e" % err, flush=True)
+    assert err < 2e-3
+
+
+@pytest.mark.parametrize("B", [128, 2048])
+def test_gradients_match_oracle(sur, B):
+    X, T = rows(B, 2 + B)
+    loss, g = sur.loss_grad(torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda())
+    lo, go = M.loss_grad(sur.get_params().astype(np.float64), X, T, emulate_bf16=True)
+    assert loss == pytest.approx(lo, rel=2e-3)
+    gb, ob = blocks(g.astype(np.float64)), blocks(go)
+    for k in gb:
+        rel = np.linalg.norm(gb[k] - ob[k]) / max(np.linalg.norm(ob[k]), 1e-30)
+        cos = np.dot(gb[k], ob[k]) / max(np.linalg.norm(gb[k]) * np.linalg.norm(ob[k]), 1e-30)
+        print("%s rel %.3e cos %.6f" % (k, rel, cos), flush=True)
+        assert rel < 3e-2 and cos > 0.999, k
+
+
+def test_features_of_texel_codes():
+    p = draw_sets(SEED_BATCH, 1)[0][0]
+    W4 = p.RES_MU_S * p.RES_NU
+    l, y, x = 3, p.RES_MU // 2 + 5, 2 * p.RES_MU_S + 7
+    code = (l * p.RES_MU + y) * W4 + x
+    e = 0x80000000 | (2 * p.SKY_W + 9)
+    X = SG.features(p, [code, e]).cpu().numpy()
+    half = p.RES_MU // 2
+    umu = 0.5 + 0.5 * (y - half) / (half - 1)
+    assert X[0, :4] == pytest.approx([2 * l / (p.RES_R - 1) - 1, 2 * umu - 1,
+                                      2 * 7 / (p.RES_MU_S - 1) - 1, 2 * 2 / (p.RES_NU - 1) - 1], abs=1e-6)
+    assert X[1, :4] == pytest.approx([2 * 2 / (p.SKY_H - 1) - 1, -2, 2 * 9 / (p.SKY_W - 1) - 1, -2], abs=1e-6)
+    assert X[0, 4] == pytest.approx((p.Rg - 5000) / 2000, abs=1e-6)
+    assert X[0, 10] == pytest.approx((p.mieG - 0.75) / 0.15, abs=1e-5)
+    assert np.all(X[:, 11:] == 0)
+
+
+def small_sets(n, seed=SEED_BATCH):
+    sets, _ = draw_sets(seed, n, res=(16, 64, 16, 4))
+    for p in sets:
+        p.TRANSMITTANCE_W, p.TRANSMITTANCE_H, p.SKY_W, p.SKY_H = 64, 16, 16, 8
+    return sets
+
+
+def test_training_reduces_loss():
+    s = SG.Surrogate(0, seed=11)
+    n = s.build_dataset(small_sets(6), options(), samples_per_set=4096)
+    assert n == 6 * (4096 + 16 * 8)
+    mu, sd, eps = s.get_norm()
+    assert np.all(sd > 0) and eps > 0
+    losses = s.train(200, batch=4096, lr=2e-3, lr_end=1e-4)
+    print("loss %.4f -> %.4f" % (losses[:5].mean(), losses[-5:].mean()), flush=True)
+    assert np.all(np.isfinite(losses))
+    assert losses[-10:].mean() < 0.3 * losses[:10].mean()
+    s.close()
+
+
+def _plain(sets, o):
+    e = L.Engine(0)
+    outs = e.precompute(sets, o)
+    torch.cuda.synchronize()
+    res = [tuple(t.double().cpu().numpy() for t in ts) for ts in outs]
+    e.close()
+    return res
+
+
+def test_judge_rejects_and_falls_back_to_plain(sur):
+    sets = small_sets(5, seed=SEED_BATCH + 1)
+    o = options()
+    outs, rep = sur.precompute(sets, o, judge_fraction=0.2, tolerance=1e-12)
+    torch.cuda.synchronize()
+    assert rep["judged"] == 1 and rep["accepted"] == 0 and rep["fallback_sets"] == 5
+    ref = _plain(sets, o)
+    for (T, E, S), (rT, rE, rS) in zip(outs, ref):
+        assert np.array_equal(T.double().cpu().numpy(), rT)
+        assert np.array_equal(E.double().cpu().numpy(), rE)
+        assert np.array_equal(S.double().cpu().numpy(), rS)
+
+
+def test_judge_accepts_and_keeps_exact_parts(sur):
+    sets = small_sets(4, seed=SEED_BATCH + 2)
+    o = options()
+    outs, rep = sur.precompute(sets, o, judge_fraction=0.5, tolerance=1e30)
+    torch.cuda.synchronize()
+    assert rep["judged"] == 2 and rep["accepted"] == 1 and rep["fallback_sets"] == 2
+    assert rep["max_rel_err_S"] >= 0 and rep["ms_mlp"] > 0
+    ref = _plain(sets, o)
+    exact = 0
+    for (T, E, S), (rT, rE, rS) in zip(outs, ref):
+        S = S.double().cpu().numpy()
+        assert np.array_equal(T.double().cpu().numpy(), rT)      # exact transmittance
+        assert np.array_equal(S[..., 3], rS[..., 3])             # exact single Mie (alpha)
+        assert np.all(np.isfinite(S)) and np.all(S >= 0)
+        exact += np.array_equal(S, rS)
+    assert exact == 2                                             # the judged sets
