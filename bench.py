@@ -135,7 +135,8 @@ def main():
     whole_tflops = F.total(p, o) / sec / 1e12
 
     extra = {"pass_ms": {k: round(v, 3) for k, v in pass_ms.items()},
-             "algorithmic_tflops_whole_precompute": round(whole_tflops, 2)}
+             "algorithmic_tflops_whole_precompute": round(whole_tflops, 2),
+             "autotuner": eng.tuning()}
     if not args.no_extra:
         # the reference configuration (Earth, 32x128x32x8, 4 orders) and its
         # accuracy at the committed FP64-oracle golden texels

@@ -85,6 +85,7 @@ typedef struct atmo_options {
 #define ATMO_FLAG_NO_GRAPH       2  /* launch passes eagerly instead of replaying a hipGraph */
 #define ATMO_FLAG_TIME_PASSES    4  /* eager launches, a hipEvent pair around each pass
                                        (read with atmo_engine_pass_times) */
+#define ATMO_FLAG_NO_AUTOTUNE    8  /* keep the default kernel variants */
 
 /* Reference defaults (PreprocessAtmo.cpp:90-98; note betaMEx = 4.44e-3). */
 ATMO_API void atmo_params_default(atmo_params *p);
@@ -151,6 +152,23 @@ typedef struct atmo_pass_time {
 } atmo_pass_time;
 ATMO_API atmo_status atmo_engine_pass_times(atmo_engine *e, atmo_pass_time *out, int capacity,
                                             int *count);
+
+/* Kernel variants the autotuner picked for the engine's current plan.  After
+ * the first run of a batch shape not seen before in the process, the engine
+ * times every variant of the two dominant passes and keeps the fastest (all
+ * variants give bit-identical tables).  Times in ms: inscatterS_ms[i] for 2 / 4
+ * texels per thread (one order-2 + (orders-2) later launches), inscatterN_ms[2c+x]
+ * for at most 1 / 2 / 4 mu rows per workgroup (c) without / with XCD-banded
+ * tile order (x). */
+typedef struct atmo_tuning {
+    int tuned;
+    int inscatterS_texels_per_thread;
+    int inscatterN_rows_per_workgroup;
+    int inscatterN_xcd_order;
+    float inscatterS_ms[2];
+    float inscatterN_ms[6];
+} atmo_tuning;
+ATMO_API atmo_status atmo_engine_tuning(atmo_engine *e, atmo_tuning *out);
 
 /* Bytes of device work memory the engine would hold for this batch. */
 ATMO_API atmo_status atmo_engine_workspace_size(const atmo_params *sets, int nsets,

@@ -91,8 +91,8 @@ typedef struct atmo_surrogate_report {
  * the plain multi-order path and compared.  If every judged set is within
  * `tolerance` (max relative error, floored at 1e-4 of the table maximum) the
  * batch is accepted (judged sets keep their plain tables); otherwise every set
- * falls back to the plain path.  Plain results are bit-identical to
- * atmo_engine_precompute. */
+ * falls back to the plain path.  judge_fraction = 0 accepts without judging.
+ * Plain results are bit-identical to atmo_engine_precompute. */
 ATMO_API atmo_status atmo_surrogate_precompute(atmo_surrogate *s, const atmo_params *sets,
                                                int nsets, const atmo_options *opts,
                                                const atmo_tables *out, float judge_fraction,

@@ -14,7 +14,10 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -193,6 +196,13 @@ struct DevBuf {
     }
 };
 
+// Kernel variants chosen by the autotuner, with the times measured for each.
+struct Tuning {
+    int jG = 4, snCap = 4, snXcd = 1;
+    float msJ[2] = {0, 0};        // G = 2, 4 (order-2 + order->=3 launch)
+    float msN[6] = {0, 0, 0, 0, 0, 0};  // caps 1, 2, 4 x (plain, XCD order)
+};
+
 struct Plan {
     std::vector<atmo_params> params;
     atmo_options opts;
@@ -203,10 +213,18 @@ struct Plan {
     size_t ldsJ1 = 0, ldsJ = 0, ldsE1 = 0, ldsEN = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    ~Plan() {
+    // kernel variants (picked by the autotuner)
+    int jG = 4;        // k_inscatterS texels per thread
+    int snCap = 4;     // k_inscatterN max mu rows per workgroup
+    int snXcd = 1;     // k_inscatterN XCD-banded tile order
+    bool tuned = false;
+    void drop_graph() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
     }
+    ~Plan() { drop_graph(); }
 };
 
 }  // namespace
@@ -225,6 +243,8 @@ struct atmo_engine {
     std::unique_ptr<Plan> plan;
     DevBuf pass_sets, pass_layers, pass_tiles, pass_dirs;
     std::vector<atmo_pass_time> times;
+    Tuning last_tuning;
+    std::string last_key;
 };
 
 namespace {
@@ -294,7 +314,8 @@ atmo_status upload_dirs(DevBuf &buf, const atmo_options &o, size_t &offJT, size_
 // Tile tables of every pass for a batch.  The spherical-integral passes list
 // the most expensive tiles first so mixed-resolution batches drain evenly.
 void build_tiles(const std::vector<atmo_params> &ps, const atmo_options &o,
-                 std::vector<uint2> &tiles, int off[P_COUNT], int cnt[P_COUNT]) {
+                 std::vector<uint2> &tiles, int off[P_COUNT], int cnt[P_COUNT], int snCap = 4,
+                 int snXcd = 1) {
     tiles.clear();
     const int n = (int)ps.size();
     for (int pass = 0; pass < P_COUNT; ++pass) {
@@ -310,9 +331,9 @@ void build_tiles(const std::vector<atmo_params> &ps, const atmo_options &o,
             case P_E1: nt = (z.nE + 255) / 256; break;
             case P_S1: nt = (z.n4 + 255) / 256; break;
             case P_SN: {
-                // workgroups of up to 4 mu rows (k_inscatterN)
+                // workgroups of up to snCap mu rows (k_inscatterN)
                 const int W4 = p.RES_MU_S * p.RES_NU;
-                const int rpw = std::max(1, std::min(4, 256 / W4));
+                const int rpw = std::max(1, std::min(snCap, 256 / W4));
                 nt = (long long)p.RES_R * ((p.RES_MU + rpw - 1) / rpw);
                 break;
             }
@@ -330,7 +351,7 @@ void build_tiles(const std::vector<atmo_params> &ps, const atmo_options &o,
                              [](const std::pair<double, uint2> &a, const std::pair<double, uint2> &b) {
                                  return a.first > b.first;
                              });
-        if (pass == P_SN) {
+        if (pass == P_SN && snXcd) {
             // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2);
             // deal them so that XCD x runs a contiguous band of mu rows, whose rays
             // gather from overlapping parts of the dJ table.
@@ -397,16 +418,154 @@ atmo_status launch_all(Plan &pl, hipStream_t st, PassTimer *tm = nullptr) {
         HIPCHK(t.begin(3, order));
         HIPCHK(launch_inscatterS(S, T + pl.off[P_J], pl.cnt[P_J], o.spherical_samples,
                                  dirs + pl.dirJT_off, dirs + pl.dirJW_off, first,
-                                 first ? pl.ldsJ1 : pl.ldsJ, st));
+                                 first ? pl.ldsJ1 : pl.ldsJ, pl.jG, st));
         HIPCHK(t.end());
         HIPCHK(t.begin(4, order));
         HIPCHK(launch_irradianceN(S, T + pl.off[P_EN], pl.cnt[P_EN], o.irradiance_samples,
                                   dirs + pl.dirE_off, first, first ? pl.ldsE1 : pl.ldsEN, st));
         HIPCHK(t.end());
         HIPCHK(t.begin(5, order));
-        HIPCHK(launch_inscatterN(S, T + pl.off[P_SN], pl.cnt[P_SN], o.inscatter_samples, st));
+        HIPCHK(launch_inscatterN(S, T + pl.off[P_SN], pl.cnt[P_SN], o.inscatter_samples, pl.snCap, st));
         HIPCHK(t.end());
     }
+    return ATMO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner.  After the first run of a plan whose shape class has not been
+// tuned in this process, every variant of the two dominant passes is timed on
+// the plan's own (by then scratch) work tables: k_inscatterS with 2 or 4
+// texels per thread (orders 2 and >= 3), k_inscatterN with up to 1, 2 or 4 mu
+// rows per workgroup, with and without the XCD-banded tile order (its
+// descriptors carry no S pointer, so the finished tables are not touched).
+// All variants compute bit-identical results; the fastest is kept per shape
+// class and the plan's graph is re-captured.
+// ---------------------------------------------------------------------------
+std::string shape_key(const std::vector<atmo_params> &ps, const atmo_options &o) {
+    std::vector<std::string> parts;
+    for (const atmo_params &p : ps) {
+        char b[96];
+        snprintf(b, sizeof b, "%d.%d.%d.%d.%d", p.RES_R, p.RES_MU, p.RES_MU_S, p.RES_NU, p.SKY_W);
+        parts.push_back(b);
+    }
+    std::sort(parts.begin(), parts.end());
+    std::string k;
+    char b[96];
+    snprintf(b, sizeof b, "NS%d.IS%d.N%zu|", o.spherical_samples, o.inscatter_samples, ps.size());
+    k = b;
+    // count of each resolution class
+    for (size_t i = 0; i < parts.size();) {
+        size_t j = i;
+        while (j < parts.size() && parts[j] == parts[i]) ++j;
+        k += parts[i] + "x" + std::to_string(j - i) + ";";
+        i = j;
+    }
+    return k;
+}
+
+std::map<std::string, Tuning> &tuning_cache() {
+    static std::map<std::string, Tuning> c;
+    return c;
+}
+std::mutex &tuning_mutex() {
+    static std::mutex m;
+    return m;
+}
+
+atmo_status time_launch(hipStream_t st, float &best, const std::function<hipError_t()> &f) {
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    best = 1e30f;
+    atmo_status rc = ATMO_OK;
+    for (int rep = 0; rep < 2 && rc == ATMO_OK; ++rep) {
+        hipError_t he = hipEventRecord(a, st);
+        if (he == hipSuccess) he = f();
+        if (he == hipSuccess) he = hipEventRecord(b, st);
+        if (he == hipSuccess) he = hipEventSynchronize(b);
+        float ms = 0.f;
+        if (he == hipSuccess) he = hipEventElapsedTime(&ms, a, b);
+        if (he != hipSuccess) rc = fail(ATMO_ERR_DEVICE, "autotune: %s", hipGetErrorString(he));
+        best = std::min(best, ms);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return rc;
+}
+
+atmo_status autotune(Plan &pl, hipStream_t st, Tuning &tu) {
+    const atmo_options &o = pl.opts;
+    const SetDesc *S = (const SetDesc *)pl.dsets.p;
+    const uint2 *T = (const uint2 *)pl.dtiles.p;
+    const char *dirs = (const char *)pl.ddirs.p;
+    HIPCHK(hipStreamSynchronize(st));
+    const int Gs[2] = {2, 4};
+    float bestJ = 1e30f;
+    for (int gi = 0; gi < 2; ++gi) {
+        float m1 = 0.f, m2 = 0.f;
+        atmo_status rc = time_launch(st, m1, [&]() {
+            return launch_inscatterS(S, T + pl.off[P_J], pl.cnt[P_J], o.spherical_samples, dirs + pl.dirJT_off,
+                                     dirs + pl.dirJW_off, true, pl.ldsJ1, Gs[gi], st);
+        });
+        if (rc != ATMO_OK) return rc;
+        rc = time_launch(st, m2, [&]() {
+            return launch_inscatterS(S, T + pl.off[P_J], pl.cnt[P_J], o.spherical_samples, dirs + pl.dirJT_off,
+                                     dirs + pl.dirJW_off, false, pl.ldsJ, Gs[gi], st);
+        });
+        if (rc != ATMO_OK) return rc;
+        // weight: one order-2 launch, (orders - 2) later ones
+        tu.msJ[gi] = m1 + m2 * std::max(0, o.orders - 2);
+        if (tu.msJ[gi] < bestJ) {
+            bestJ = tu.msJ[gi];
+            tu.jG = Gs[gi];
+        }
+    }
+    // k_inscatterN variants: descriptors without S (the accumulation is skipped)
+    std::vector<SetDesc> hs = pl.hsets;
+    for (auto &d : hs) d.S4 = nullptr;
+    DevBuf ds, dt;
+    HIPCHK(ds.ensure(hs.size() * sizeof(SetDesc)));
+    HIPCHK(hipMemcpy(ds.p, hs.data(), hs.size() * sizeof(SetDesc), hipMemcpyHostToDevice));
+    const int caps[3] = {1, 2, 4};
+    float bestN = 1e30f;
+    for (int ci = 0; ci < 3; ++ci)
+        for (int xo = 0; xo < 2; ++xo) {
+            std::vector<uint2> tiles;
+            int off[P_COUNT], cnt[P_COUNT];
+            build_tiles(pl.params, o, tiles, off, cnt, caps[ci], xo);
+            HIPCHK(dt.ensure(tiles.size() * sizeof(uint2)));
+            HIPCHK(hipMemcpy(dt.p, tiles.data(), tiles.size() * sizeof(uint2), hipMemcpyHostToDevice));
+            float ms = 0.f;
+            atmo_status rc = time_launch(st, ms, [&]() {
+                return launch_inscatterN((const SetDesc *)ds.p, (const uint2 *)dt.p + off[P_SN], cnt[P_SN],
+                                         o.inscatter_samples, caps[ci], st);
+            });
+            if (rc != ATMO_OK) return rc;
+            tu.msN[ci * 2 + xo] = ms;
+            if (ms < bestN) {
+                bestN = ms;
+                tu.snCap = caps[ci];
+                tu.snXcd = xo;
+            }
+        }
+    return ATMO_OK;
+}
+
+// Applies a tuning to a plan: rebuilds the tile table if the k_inscatterN
+// variant changed and drops the captured graph.
+atmo_status apply_tuning(Plan &pl, const Tuning &tu) {
+    const bool retile = tu.snCap != pl.snCap || tu.snXcd != pl.snXcd;
+    pl.jG = tu.jG;
+    pl.snCap = tu.snCap;
+    pl.snXcd = tu.snXcd;
+    pl.tuned = true;
+    if (retile) {
+        std::vector<uint2> tiles;
+        build_tiles(pl.params, pl.opts, tiles, pl.off, pl.cnt, pl.snCap, pl.snXcd);
+        HIPCHK(pl.dtiles.ensure(tiles.size() * sizeof(uint2)));
+        HIPCHK(hipMemcpy(pl.dtiles.p, tiles.data(), tiles.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    }
+    pl.drop_graph();
     return ATMO_OK;
 }
 
@@ -553,6 +712,18 @@ atmo_status atmo_engine_pass_times(atmo_engine *e, atmo_pass_time *out, int capa
     return ATMO_OK;
 }
 
+atmo_status atmo_engine_tuning(atmo_engine *e, atmo_tuning *out) {
+    if (!e || !out) return fail(ATMO_ERR_INVALID_ARGUMENT, "bad arguments");
+    memset(out, 0, sizeof *out);
+    out->tuned = !e->last_key.empty();
+    out->inscatterS_texels_per_thread = e->last_tuning.jG;
+    out->inscatterN_rows_per_workgroup = e->last_tuning.snCap;
+    out->inscatterN_xcd_order = e->last_tuning.snXcd;
+    for (int i = 0; i < 2; ++i) out->inscatterS_ms[i] = e->last_tuning.msJ[i];
+    for (int i = 0; i < 6; ++i) out->inscatterN_ms[i] = e->last_tuning.msN[i];
+    return ATMO_OK;
+}
+
 atmo_status atmo_engine_workspace_size(const atmo_params *sets, int nsets, const atmo_options *opts,
                                        size_t *bytes) {
     if (!sets || nsets <= 0 || !bytes) return fail(ATMO_ERR_INVALID_ARGUMENT, "bad arguments");
@@ -627,6 +798,27 @@ atmo_status atmo_engine_precompute(atmo_engine *e, const atmo_params *sets, int 
             HIPCHK(hipGraphInstantiate(&pl.gexec, g, nullptr, nullptr, 0));
         }
         HIPCHK(hipGraphLaunch(pl.gexec, hs));
+    }
+    if (!pl.tuned && (o.orders >= 2) && !(o.flags & ATMO_FLAG_NO_AUTOTUNE)) {
+        const std::string key = shape_key(pl.params, o);
+        Tuning tu;
+        bool have = false;
+        {
+            std::lock_guard<std::mutex> lk(tuning_mutex());
+            auto it = tuning_cache().find(key);
+            if (it != tuning_cache().end()) {
+                tu = it->second;
+                have = true;
+            }
+        }
+        if (!have) {
+            if ((st = autotune(pl, hs, tu)) != ATMO_OK) return st;
+            std::lock_guard<std::mutex> lk(tuning_mutex());
+            tuning_cache()[key] = tu;
+        }
+        if ((st = apply_tuning(pl, tu)) != ATMO_OK) return st;
+        e->last_tuning = tu;
+        e->last_key = key;
     }
     if (o.flags & ATMO_FLAG_CHECK_NUMERICS) {
         HIPCHK(hipMemsetAsync(pl.dflags.p, 0, nsets * sizeof(unsigned), hs));
@@ -719,7 +911,7 @@ static atmo_status pass_run(atmo_engine *e, const atmo_params *p, const atmo_opt
         size_t lds = lds_inscatterS(o.spherical_samples, p->RES_NU, p->SKY_W, first);
         if (lds > 160 * 1024) return fail(ATMO_ERR_UNSUPPORTED, "LDS staging too large");
         HIPCHK(launch_inscatterS(S, T + off[P_J], cnt[P_J], o.spherical_samples, d + oJT,
-                                 d + oJW, first, lds, hs));
+                                 d + oJW, first, lds, 4, hs));
         break;
     }
     case P_EN: {
@@ -728,7 +920,7 @@ static atmo_status pass_run(atmo_engine *e, const atmo_params *p, const atmo_opt
                                   first, lds, hs));
         break;
     }
-    case P_SN: HIPCHK(launch_inscatterN(S, T + off[P_SN], cnt[P_SN], o.inscatter_samples, hs)); break;
+    case P_SN: HIPCHK(launch_inscatterN(S, T + off[P_SN], cnt[P_SN], o.inscatter_samples, 4, hs)); break;
     }
     HIPCHK(hipStreamSynchronize(hs));
     g_err.clear();

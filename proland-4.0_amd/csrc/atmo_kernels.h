@@ -24,13 +24,16 @@ hipError_t launch_transmittance(const SetDesc *sets, const uint2 *tiles, int nti
 hipError_t launch_irradiance1(const SetDesc *sets, const uint2 *tiles, int ntiles, hipStream_t st);
 hipError_t launch_inscatter1(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
                              hipStream_t st);
+// G: texels per thread (2 or 4)
 hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS,
-                             const void *dirT, const void *dirW, bool first, size_t lds,
+                             const void *dirT, const void *dirW, bool first, size_t lds, int G,
                              hipStream_t st);
 hipError_t launch_irradianceN(const SetDesc *sets, const uint2 *tiles, int ntiles, int NI,
                               const void *dirs, bool first, size_t lds, hipStream_t st);
+// rpw_cap: at most this many mu rows per workgroup (1, 2 or 4); the tile table
+// must have been built with the same cap (sn_tiles_per_set)
 hipError_t launch_inscatterN(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
-                             hipStream_t st);
+                             int rpw_cap, hipStream_t st);
 hipError_t launch_check(const SetDesc *sets, const uint2 *tiles, int ntiles, unsigned int *flags,
                         hipStream_t st);
 

@@ -200,18 +200,12 @@ struct DirJ {      // per-theta constants of the NS x 2NS grid
     float ct, st, dw, pad;
 };
 
-#ifndef ATMO_JG
-#define ATMO_JG 4
-#endif
-constexpr int kJG = ATMO_JG;  // texels per thread in k_inscatterS
-
-template <bool FIRST>
+template <bool FIRST, int G>
 __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ sets,
                                                     const uint2 *__restrict__ tiles, int NS,
                                                     const DirJ *__restrict__ dirT,
                                                     const float2 *__restrict__ dirW) {
     constexpr int NT = FIRST ? 2 : 1;
-    constexpr int G = kJG;
     extern __shared__ float4 lds[];
     const uint2 tl = tiles[blockIdx.x];
     const SetDesc &s = sets[tl.x];
@@ -561,7 +555,10 @@ __global__ __launch_bounds__(256) void k_irradianceN(const SetDesc *__restrict__
 constexpr int kNChunk = 64;   // samples staged per chunk
 constexpr int kNRows = 4;     // max mu rows per workgroup
 
-__device__ __forceinline__ int sn_rows_per_wg(int W4) { return max(1, min(kNRows, 256 / W4)); }
+// rows per workgroup: as many whole rows as fit 256 threads, at most rpw_cap (<= 4)
+__device__ __forceinline__ int sn_rows_per_wg(int W4, int rpw_cap) {
+    return max(1, min(min(rpw_cap, kNRows), 256 / W4));
+}
 
 struct SnEntry {
     float4 w;     // bilinear (r, mu) weights of the 4 table rows
@@ -572,12 +569,12 @@ struct SnEntry {
 
 __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ sets,
                                                     const uint2 *__restrict__ tiles,
-                                                    int nsamp) {
+                                                    int nsamp, int rpw_cap) {
     __shared__ SnEntry ent[kNRows][kNChunk];
     __shared__ float rowdx[kNRows];
     const uint2 tl = tiles[blockIdx.x];
     const SetDesc &s = sets[tl.x];
-    const int RPW = sn_rows_per_wg(s.W4);
+    const int RPW = sn_rows_per_wg(s.W4, rpw_cap);
     const int ngy = (s.RMU + RPW - 1) / RPW;
     const int l = (int)tl.y / ngy;
     const int yb = ((int)tl.y % ngy) * RPW;
@@ -811,15 +808,20 @@ hipError_t launch_inscatter1(const SetDesc *sets, const uint2 *tiles, int ntiles
     return hipGetLastError();
 }
 hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS,
-                             const void *dirT, const void *dirW, bool first, size_t lds,
+                             const void *dirT, const void *dirW, bool first, size_t lds, int G,
                              hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
-    if (first)
-        hipLaunchKernelGGL(k_inscatterS<true>, dim3(ntiles), dim3(256), lds, st, sets, tiles, NS,
-                           (const DirJ *)dirT, (const float2 *)dirW);
-    else
-        hipLaunchKernelGGL(k_inscatterS<false>, dim3(ntiles), dim3(256), lds, st, sets, tiles, NS,
-                           (const DirJ *)dirT, (const float2 *)dirW);
+#define ATMO_LJ(F, GG)                                                                          \
+    hipLaunchKernelGGL((k_inscatterS<F, GG>), dim3(ntiles), dim3(256), lds, st, sets, tiles, NS, \
+                       (const DirJ *)dirT, (const float2 *)dirW)
+    if (G == 2) {
+        if (first) ATMO_LJ(true, 2); else ATMO_LJ(false, 2);
+    } else if (G == 4) {
+        if (first) ATMO_LJ(true, 4); else ATMO_LJ(false, 4);
+    } else {
+        return hipErrorInvalidValue;
+    }
+#undef ATMO_LJ
     return hipGetLastError();
 }
 hipError_t launch_irradianceN(const SetDesc *sets, const uint2 *tiles, int ntiles, int NI,
@@ -834,9 +836,9 @@ hipError_t launch_irradianceN(const SetDesc *sets, const uint2 *tiles, int ntile
     return hipGetLastError();
 }
 hipError_t launch_inscatterN(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
-                             hipStream_t st) {
+                             int rpw_cap, hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_inscatterN, dim3(ntiles), dim3(256), 0, st, sets, tiles, nsamp);
+    hipLaunchKernelGGL(k_inscatterN, dim3(ntiles), dim3(256), 0, st, sets, tiles, nsamp, rpw_cap);
     return hipGetLastError();
 }
 hipError_t launch_check(const SetDesc *sets, const uint2 *tiles, int ntiles, unsigned int *flags,

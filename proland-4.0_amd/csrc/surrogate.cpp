@@ -549,7 +549,8 @@ atmo_status atmo_surrogate_precompute(atmo_surrogate *s, const atmo_params *sets
     r.judged = nj;
     r.max_rel_err_S = worstS;
     r.max_rel_err_E = worstE;
-    r.accepted = nj > 0 && worstS <= tolerance && worstE <= tolerance;
+    // no judged set (judge_fraction = 0): the network is trusted as is
+    r.accepted = nj == 0 || (worstS <= tolerance && worstE <= tolerance);
     if (r.accepted) {
         // judged sets keep their exact plain tables
         for (int k = 0; k < nj; ++k) {

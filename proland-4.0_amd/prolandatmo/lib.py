@@ -30,6 +30,19 @@ PASS_NAMES = ("transmittance", "irradiance1", "inscatter1", "inscatterS", "irrad
               "inscatterN")
 
 
+class AtmoTuning(ctypes.Structure):
+    _fields_ = [("tuned", ctypes.c_int), ("inscatterS_texels_per_thread", ctypes.c_int),
+                ("inscatterN_rows_per_workgroup", ctypes.c_int), ("inscatterN_xcd_order", ctypes.c_int),
+                ("inscatterS_ms", ctypes.c_float * 2), ("inscatterN_ms", ctypes.c_float * 6)]
+
+    def as_dict(self):
+        return {"tuned": self.tuned, "inscatterS_texels_per_thread": self.inscatterS_texels_per_thread,
+                "inscatterN_rows_per_workgroup": self.inscatterN_rows_per_workgroup,
+                "inscatterN_xcd_order": self.inscatterN_xcd_order,
+                "inscatterS_ms": [round(x, 3) for x in self.inscatterS_ms],
+                "inscatterN_ms": [round(x, 3) for x in self.inscatterN_ms]}
+
+
 class AtmoError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__("%s: %s" % (STATUS.get(status, status), msg))
@@ -57,6 +70,7 @@ SIGNATURES = {
                                               ctypes.POINTER(AtmoTables), _vp]),
     "atmo_engine_pass_times": (ctypes.c_int, [_vp, ctypes.POINTER(AtmoPassTime), ctypes.c_int,
                                               ctypes.POINTER(ctypes.c_int)]),
+    "atmo_engine_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(AtmoTuning)]),
     "atmo_engine_workspace_size": (ctypes.c_int, [_pp, ctypes.c_int, _po,
                                                   ctypes.POINTER(ctypes.c_size_t)]),
     "atmo_pass_transmittance": (ctypes.c_int, [_vp, _pp, _po, _vp, _vp]),
@@ -173,6 +187,11 @@ class Engine:
         buf = (AtmoPassTime * max(n.value, 1))()
         check(lib().atmo_engine_pass_times(self.h, buf, n.value, ctypes.byref(n)))
         return [(PASS_NAMES[buf[i].pass_], buf[i].order, buf[i].ms) for i in range(n.value)]
+
+    def tuning(self):
+        t = AtmoTuning()
+        check(lib().atmo_engine_tuning(self.h, ctypes.byref(t)))
+        return t.as_dict()
 
     # ---- pass level (float4 device tables) ----
     def pass_transmittance(self, p, opts, T4, stream=None):

@@ -70,6 +70,7 @@ assert ctypes.sizeof(AtmoOptions) == 64
 FLAG_CHECK_NUMERICS = 1
 FLAG_NO_GRAPH = 2
 FLAG_TIME_PASSES = 4
+FLAG_NO_AUTOTUNE = 8
 
 
 def earth(**over):

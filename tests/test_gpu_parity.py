@@ -258,3 +258,22 @@ def test_preprocess_writes_reference_files(eng, tmp_path):
     m = os.path.getmtime(os.path.join(d, "inscatter.raw"))
     L.preprocess(p, d)  # cached: returns without recomputing
     assert os.path.getmtime(os.path.join(d, "inscatter.raw")) == m
+
+
+def test_autotuned_variants_give_identical_tables():
+    p = small(res=(8, 32, 16, 8))
+    e1 = L.Engine(0)
+    ref = e1.precompute(p, options(flags=8))  # default variants, no tuning
+    torch.cuda.synchronize()
+    e2 = L.Engine(0)
+    outs = e2.precompute(p, options())       # first run, then the autotuner
+    outs = e2.precompute(p, options(), outs=outs)  # tuned variants, re-captured graph
+    torch.cuda.synchronize()
+    t = e2.tuning()
+    print(t, flush=True)
+    assert t["tuned"] == 1 and t["inscatterS_texels_per_thread"] in (2, 4)
+    assert t["inscatterN_rows_per_workgroup"] in (1, 2, 4)
+    for a, b in zip(ref[0], outs[0]):
+        assert torch.equal(a, b)
+    e1.close()
+    e2.close()
