@@ -78,7 +78,10 @@ typedef struct atmo_options {
     int irradiance_samples;     /* IRRADIANCE_INTEGRAL_SAMPLES (32) */
     int spherical_samples;      /* INSCATTER_SPHERICAL_INTEGRAL_SAMPLES (16) */
     int flags;                  /* ATMO_FLAG_* */
-    int reserved[10];
+    int texels_per_thread;      /* k_inscatterS variant 2, 4 or 8 (bit-identical results);
+                                   0 = 4 for the pass-level entry points, autotuned in
+                                   atmo_engine_precompute */
+    int reserved[9];
 } atmo_options;
 
 #define ATMO_FLAG_CHECK_NUMERICS 1  /* validate outputs (finite, T in [0,1], E,S >= 0) */
@@ -156,7 +159,7 @@ ATMO_API atmo_status atmo_engine_pass_times(atmo_engine *e, atmo_pass_time *out,
 /* Kernel variants the autotuner picked for the engine's current plan.  After
  * the first run of a batch shape not seen before in the process, the engine
  * times every variant of the two dominant passes and keeps the fastest (all
- * variants give bit-identical tables).  Times in ms: inscatterS_ms[i] for 2 / 4
+ * variants give bit-identical tables).  Times in ms: inscatterS_ms[i] for 2 / 4 / 8
  * texels per thread (one order-2 + (orders-2) later launches), inscatterN_ms[2c+x]
  * for at most 1 / 2 / 4 mu rows per workgroup (c) without / with XCD-banded
  * tile order (x). */
@@ -165,7 +168,7 @@ typedef struct atmo_tuning {
     int inscatterS_texels_per_thread;
     int inscatterN_rows_per_workgroup;
     int inscatterN_xcd_order;
-    float inscatterS_ms[2];
+    float inscatterS_ms[3];
     float inscatterN_ms[6];
 } atmo_tuning;
 ATMO_API atmo_status atmo_engine_tuning(atmo_engine *e, atmo_tuning *out);

@@ -102,6 +102,9 @@ atmo_status validate_opts(const atmo_options &o) {
         return fail(ATMO_ERR_INVALID_ARGUMENT, "irradiance_samples must be even, in [2, 256]");
     if (o.spherical_samples < 1 || o.spherical_samples > 128)
         return fail(ATMO_ERR_INVALID_ARGUMENT, "spherical_samples must be in [1, 128]");
+    if (o.texels_per_thread != 0 && o.texels_per_thread != 2 && o.texels_per_thread != 4 &&
+        o.texels_per_thread != 8)
+        return fail(ATMO_ERR_INVALID_ARGUMENT, "texels_per_thread must be 0, 2, 4 or 8");
     return ATMO_OK;
 }
 
@@ -176,7 +179,7 @@ bool same_params(const atmo_params &a, const atmo_params &b) { return memcmp(&a,
 bool same_opts(const atmo_options &a, const atmo_options &b) {
     return a.orders == b.orders && a.transmittance_samples == b.transmittance_samples &&
            a.inscatter_samples == b.inscatter_samples && a.irradiance_samples == b.irradiance_samples &&
-           a.spherical_samples == b.spherical_samples;
+           a.spherical_samples == b.spherical_samples && a.texels_per_thread == b.texels_per_thread;
 }
 
 struct DevBuf {
@@ -199,7 +202,7 @@ struct DevBuf {
 // Kernel variants chosen by the autotuner, with the times measured for each.
 struct Tuning {
     int jG = 4, snCap = 4, snXcd = 1;
-    float msJ[2] = {0, 0};        // G = 2, 4 (order-2 + order->=3 launch)
+    float msJ[3] = {0, 0, 0};     // G = 2, 4, 8 (order-2 + order->=3 launches)
     float msN[6] = {0, 0, 0, 0, 0, 0};  // caps 1, 2, 4 x (plain, XCD order)
 };
 
@@ -499,9 +502,9 @@ atmo_status autotune(Plan &pl, hipStream_t st, Tuning &tu) {
     const uint2 *T = (const uint2 *)pl.dtiles.p;
     const char *dirs = (const char *)pl.ddirs.p;
     HIPCHK(hipStreamSynchronize(st));
-    const int Gs[2] = {2, 4};
+    const int Gs[3] = {2, 4, 8};
     float bestJ = 1e30f;
-    for (int gi = 0; gi < 2; ++gi) {
+    for (int gi = 0; gi < 3; ++gi) {
         float m1 = 0.f, m2 = 0.f;
         atmo_status rc = time_launch(st, m1, [&]() {
             return launch_inscatterS(S, T + pl.off[P_J], pl.cnt[P_J], o.spherical_samples, dirs + pl.dirJT_off,
@@ -555,7 +558,7 @@ atmo_status autotune(Plan &pl, hipStream_t st, Tuning &tu) {
 // variant changed and drops the captured graph.
 atmo_status apply_tuning(Plan &pl, const Tuning &tu) {
     const bool retile = tu.snCap != pl.snCap || tu.snXcd != pl.snXcd;
-    pl.jG = tu.jG;
+    pl.jG = pl.opts.texels_per_thread ? pl.opts.texels_per_thread : tu.jG;
     pl.snCap = tu.snCap;
     pl.snXcd = tu.snXcd;
     pl.tuned = true;
@@ -574,6 +577,7 @@ atmo_status build_plan(atmo_engine *e, const atmo_params *sets, int nsets, const
     std::unique_ptr<Plan> pl(new Plan);
     pl->params.assign(sets, sets + nsets);
     pl->opts = o;
+    pl->jG = o.texels_per_thread ? o.texels_per_thread : 4;
     // work arena
     size_t total = 0;
     std::vector<size_t> base(nsets);
@@ -719,7 +723,7 @@ atmo_status atmo_engine_tuning(atmo_engine *e, atmo_tuning *out) {
     out->inscatterS_texels_per_thread = e->last_tuning.jG;
     out->inscatterN_rows_per_workgroup = e->last_tuning.snCap;
     out->inscatterN_xcd_order = e->last_tuning.snXcd;
-    for (int i = 0; i < 2; ++i) out->inscatterS_ms[i] = e->last_tuning.msJ[i];
+    for (int i = 0; i < 3; ++i) out->inscatterS_ms[i] = e->last_tuning.msJ[i];
     for (int i = 0; i < 6; ++i) out->inscatterN_ms[i] = e->last_tuning.msN[i];
     return ATMO_OK;
 }
@@ -911,7 +915,7 @@ static atmo_status pass_run(atmo_engine *e, const atmo_params *p, const atmo_opt
         size_t lds = lds_inscatterS(o.spherical_samples, p->RES_NU, p->SKY_W, first);
         if (lds > 160 * 1024) return fail(ATMO_ERR_UNSUPPORTED, "LDS staging too large");
         HIPCHK(launch_inscatterS(S, T + off[P_J], cnt[P_J], o.spherical_samples, d + oJT,
-                                 d + oJW, first, lds, 4, hs));
+                                 d + oJW, first, lds, o.texels_per_thread ? o.texels_per_thread : 4, hs));
         break;
     }
     case P_EN: {

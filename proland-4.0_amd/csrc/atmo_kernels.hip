@@ -192,9 +192,13 @@ __device__ __forceinline__ void stage_slice(const SetDesc &s, const float4 *cons
 //
 // Each thread integrates G texels of one mu row (G nu slices): they share
 // the view vector v, so the phase factors of nu2 = v.w are evaluated once per
-// direction for all G.  Per direction and texel the lookup position is
-// pos = (nu1 + 1)/2 (RNU - 1) + 1 = posA s'.w + posB, with s' = posA s folded
-// in per texel and the mu_s term folded in per theta.
+// direction for all G, and since v has no y component the directions phi and
+// 2 pi - phi share them too; the thread walks the grid in such mirrored pairs
+// and folds both phases and the scattering coefficients into one rgb weight
+// per pair, as the reference's sum does per direction.  Per direction and
+// texel the lookup position is pos = (nu1 + 1)/2 (RNU - 1) + 1 =
+// posA s'.w + posB, with s' = posA s folded in per texel and the mu_s term
+// folded in per theta.
 // ---------------------------------------------------------------------------
 struct DirJ {      // per-theta constants of the NS x 2NS grid
     float ct, st, dw, pad;
@@ -307,7 +311,6 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
     const float iposA = (float)(1.0 / posAd), nposBi = (float)(-posBd / posAd);
     const float g2p1 = s.g2p1, m2g = s.m2g;
     const float SWm1 = (float)(s.SW - 1);
-    const unsigned NUu = (unsigned)NU;
     const int ngk = (NU + G - 1) / G;
     const int ngroups = s.RMU * ngk;
 
@@ -333,11 +336,49 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
             syp[g] = (float)(posAd * syd);
         }
         const float cx1 = m2g * vx;
-        float aR[G][3], aM[G][3];
+        float acc[G][3];
 #pragma unroll
         for (int g = 0; g < G; ++g)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) aR[g][c] = aM[g][c] = 0.f;
+            for (int c = 0; c < 3; ++c) acc[g][c] = 0.f;
+
+        // 4D lookup of one direction from the staged pairs: the clamp of the
+        // nu slice is implicit, |s.w| <= 1 keeps pos in [1, RNU] (+ rounding)
+        // and kk = trunc(pos) in [0, RNU] = the staged pair range.
+        auto lookup = [&](const float4 *pt, float pos, float out[3]) {
+            const unsigned kk = __float2uint_rz(pos);
+            const float f = __builtin_amdgcn_fractf(pos);
+            const float4 *pp = pt + kk * (NT * 2);
+            const float4 a = pp[0];
+            const float2 d = *(const float2 *)(pp + 1);
+            out[0] = fmaf(f, a.w, a.x);
+            out[1] = fmaf(f, d.x, a.y);
+            out[2] = fmaf(f, d.y, a.z);
+            if (FIRST) {  // order 2: Rayleigh and Mie weighted by the phase of nu1 = s.w
+                const float4 am = pp[2];
+                const float2 dm = *(const float2 *)(pp + 3);
+                const float m0 = fmaf(f, am.w, am.x), m1 = fmaf(f, dm.x, am.y),
+                            m2 = fmaf(f, dm.y, am.z);
+                const float nu1 = fmaf(pos, iposA, nposBi);
+                const float q1 = fmaf(nu1, nu1, 1.0f);
+                const float y1 = __builtin_amdgcn_rsqf(fmaf(m2g, nu1, g2p1));
+                const float t1 = q1 * (y1 * y1 * y1);
+                out[0] = fmaf(out[0], q1, m0 * t1);
+                out[1] = fmaf(out[1], q1, m1 * t1);
+                out[2] = fmaf(out[2], q1, m2 * t1);
+            }
+        };
+        // irradiance at the ground point seen in direction w, row 0 of dE
+        auto ground_e = [&](float pos, float cg1, float cg0, float out[3]) {
+            const float xg = __builtin_amdgcn_fmed3f(fmaf(pos, cg1, cg0), 0.0f, SWm1);
+            const unsigned kg = __float2uint_rz(xg);
+            const float fg = __builtin_amdgcn_fractf(xg);
+            const float4 ea = epair[2 * kg];
+            const float2 ed = *(const float2 *)(epair + 2 * kg + 1);
+            out[0] = fmaf(fg, ea.w, ea.x);
+            out[1] = fmaf(fg, ed.x, ea.y);
+            out[2] = fmaf(fg, ed.y, ea.z);
+        };
 
         for (int it = 0; it < NS; ++it) {
             const float ct = dirT[it].ct;
@@ -349,60 +390,46 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
             const float4 *pt = pairs + (size_t)it * NP * NT * 2;
             const float2 *wrow = dirW + (size_t)it * 2 * NS;
             float2 wn = wrow[0];
-            for (int ip = 0; ip < 2 * NS; ++ip) {
+            // directions phi and 2 pi - phi together: same w.x, opposite w.y,
+            // hence the same nu2 = v.w (v has no y component) and weights
+            for (int ip = 0; ip < NS; ++ip) {
                 const float2 w = wn;
-                if (ip + 1 < 2 * NS) wn = wrow[ip + 1];  // scalar prefetch of the next direction
-                // phase factors of nu2 = v.w, shared by the G texels (dw folded in)
+                if (ip + 1 < NS) wn = wrow[ip + 1];  // scalar prefetch of the next direction
                 const float nu2 = fmaf(vx, w.x, vzwz);
                 const float q2 = fmaf(nu2 * dw, nu2, dw);
                 const float y2 = __builtin_amdgcn_rsqf(fmaf(cx1, w.x, cx0));
                 const float t2 = q2 * (y2 * y2 * y2);
-                float pos[G], lv[G][3];
+                // betaR e^(-h/HR) pR(nu2) dw + betaMSca e^(-h/HM) pM(nu2) dw (:182-185)
+                const float W0 = fmaf(A0, q2, B0 * t2), W1 = fmaf(A1, q2, B1 * t2),
+                            W2 = fmaf(A2, q2, B2 * t2);
+                float pa[G], pb[G], lv[G][3];
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
-                    pos[g] = fmaf(sxp[g], w.x, fmaf(syp[g], w.y, cp));
-                    const unsigned kk = min(__float2uint_rz(pos[g]), NUu);
-                    const float f = __builtin_amdgcn_fractf(pos[g]);
-                    const float4 *pp = pt + kk * (NT * 2);
-                    const float4 a = pp[0];
-                    const float2 d = *(const float2 *)(pp + 1);
-                    lv[g][0] = fmaf(f, a.w, a.x);
-                    lv[g][1] = fmaf(f, d.x, a.y);
-                    lv[g][2] = fmaf(f, d.y, a.z);
-                    if (FIRST) {
-                        const float4 am = pp[2];
-                        const float2 dm = *(const float2 *)(pp + 3);
-                        const float m0 = fmaf(f, am.w, am.x), m1 = fmaf(f, dm.x, am.y),
-                                    m2 = fmaf(f, dm.y, am.z);
-                        const float nu1 = fmaf(pos[g], iposA, nposBi);
-                        const float q1 = fmaf(nu1, nu1, 1.0f);
-                        const float y1 = __builtin_amdgcn_rsqf(fmaf(m2g, nu1, g2p1));
-                        const float t1 = q1 * (y1 * y1 * y1);
-                        lv[g][0] = fmaf(lv[g][0], q1, m0 * t1);
-                        lv[g][1] = fmaf(lv[g][1], q1, m1 * t1);
-                        lv[g][2] = fmaf(lv[g][2], q1, m2 * t1);
-                    }
+                    const float base = fmaf(sxp[g], w.x, cp);
+                    pa[g] = fmaf(syp[g], w.y, base);
+                    pb[g] = fmaf(-syp[g], w.y, base);
+                    float la[3], lb[3];
+                    lookup(pt, pa[g], la);
+                    lookup(pt, pb[g], lb);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) lv[g][c] = la[c] + lb[c];
                 }
                 if (ground) {
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
-                        const float xg = __builtin_amdgcn_fmed3f(fmaf(pos[g], g1.x, g1.y), 0.0f, SWm1);
-                        const unsigned kg = __float2uint_rz(xg);
-                        const float fg = __builtin_amdgcn_fractf(xg);
-                        const float4 ea = epair[2 * kg];
-                        const float2 ed = *(const float2 *)(epair + 2 * kg + 1);
-                        lv[g][0] = fmaf(g0.x, fmaf(fg, ea.w, ea.x), lv[g][0]);
-                        lv[g][1] = fmaf(g0.y, fmaf(fg, ed.x, ea.y), lv[g][1]);
-                        lv[g][2] = fmaf(g0.z, fmaf(fg, ed.y, ea.z), lv[g][2]);
+                        float ea[3], eb[3];
+                        ground_e(pa[g], g1.x, g1.y, ea);
+                        ground_e(pb[g], g1.x, g1.y, eb);
+                        lv[g][0] = fmaf(g0.x, ea[0] + eb[0], lv[g][0]);
+                        lv[g][1] = fmaf(g0.y, ea[1] + eb[1], lv[g][1]);
+                        lv[g][2] = fmaf(g0.z, ea[2] + eb[2], lv[g][2]);
                     }
                 }
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        aR[g][c] = fmaf(lv[g][c], q2, aR[g][c]);
-                        aM[g][c] = fmaf(lv[g][c], t2, aM[g][c]);
-                    }
+                    acc[g][0] = fmaf(lv[g][0], W0, acc[g][0]);
+                    acc[g][1] = fmaf(lv[g][1], W1, acc[g][1]);
+                    acc[g][2] = fmaf(lv[g][2], W2, acc[g][2]);
                 }
             }
         }
@@ -411,9 +438,7 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
             const int k = kb + g;
             if (k < NU) {
                 const size_t o = ((size_t)l * s.RMU + y) * s.W4 + (size_t)k * s.RMUS + m;
-                s.dJ4[o] = make_float4(fmaf(A0, aR[g][0], B0 * aM[g][0]),
-                                       fmaf(A1, aR[g][1], B1 * aM[g][1]),
-                                       fmaf(A2, aR[g][2], B2 * aM[g][2]), 0.f);
+                s.dJ4[o] = make_float4(acc[g][0], acc[g][1], acc[g][2], 0.f);
             }
         }
     }
@@ -818,6 +843,8 @@ hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles
         if (first) ATMO_LJ(true, 2); else ATMO_LJ(false, 2);
     } else if (G == 4) {
         if (first) ATMO_LJ(true, 4); else ATMO_LJ(false, 4);
+    } else if (G == 8) {
+        if (first) ATMO_LJ(true, 8); else ATMO_LJ(false, 8);
     } else {
         return hipErrorInvalidValue;
     }
