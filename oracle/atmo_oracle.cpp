@@ -704,4 +704,192 @@ void oracle_precompute(const atmo_params *a, const atmo_options *o, double *T, d
     }
 }
 
+
+/* ------------------------------------------------------------------------
+ * Table consumer: the public functions of src/atmo/examples/atmo/
+ * atmosphereShader.glsl (built with OPTIMIZE, ATMO_FULL, HORIZON_HACK,
+ * TRANSMITTANCE_NON_LINEAR and INSCATTER_NON_LINEAR as declared at :62-64,
+ * :127-128) reading the tables this preprocess writes.  The shader hard-codes
+ * Earth in metres (:72-125); here every constant comes from atmo_params, and
+ * lengths are in the parameters' unit (km for Earth), so the shader's 600 m
+ * near-ground offset (:472) is 600 / meters_per_unit.
+ * Deviation C1: with OPTIMIZE, SQRT(f, err) is sqrt(f) (:324-330), NaN for a
+ * ray that misses the top sphere from outside; GLSL max(NaN, 0) is undefined
+ * and GPUs return 0, so din = 0 there.
+ * ------------------------------------------------------------------------ */
+static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static const double kSunIntensity = 100.0; /* atmosphereShader.glsl:66 */
+
+/* :300-302 getMie */
+static void get_mie(const P &p, const double rm[4], double out[3]) {
+    for (int c = 0; c < 3; ++c) out[c] = rm[c] * rm[3] / std::max(rm[0], 1e-4) * (p.bR[0] / p.bR[c]);
+}
+
+static void transmittance_ratio(const P &p, const double *T, double r0, double mu0, double r,
+                                double mu, double out[3]) {
+    double a[3], b[3];
+    if (mu > 0.0) {
+        T_lookup(p, T, r0, mu0, a);
+        T_lookup(p, T, r, mu, b);
+    } else {
+        T_lookup(p, T, r, -mu, a);
+        T_lookup(p, T, r0, -mu0, b);
+    }
+    for (int c = 0; c < 3; ++c) out[c] = std::min(a[c] / b[c], 1.0);
+}
+
+/* :371-418 skyRadiance */
+void oracle_sky_radiance(const atmo_params *a, const double *T, const double *S,
+                         const double camera[3], const double viewdir[3],
+                         const double sundir[3], double shaft_width, double rad[3],
+                         double ext[3]) {
+    P p = mk(a, nullptr);
+    double cam[3];
+    for (int c = 0; c < 3; ++c) cam[c] = camera[c] + viewdir[c] * std::max(shaft_width, 0.0);
+    double r = std::sqrt(dot3(cam, cam));
+    double rMu = dot3(cam, viewdir);
+    double mu = rMu / r;
+    const double r0 = r, mu0 = mu;
+    const double q = rMu * rMu - r * r + p.Rt * p.Rt;
+    const double din = q >= 0.0 ? std::max(-rMu - std::sqrt(q), 0.0) : 0.0; /* C1 */
+    if (din > 0.0) {
+        for (int c = 0; c < 3; ++c) cam[c] += din * viewdir[c];
+        rMu += din;
+        mu = rMu / p.Rt;
+        r = p.Rt;
+    }
+    if (r <= p.Rt) {
+        const double nu = dot3(viewdir, sundir);
+        const double muS = dot3(cam, sundir) / r;
+        double is[4];
+        tex4D(p, S, 4, r, rMu / r, muS, nu, is);
+        if (shaft_width > 0.0) {
+            double f[3];
+            transmittance_ratio(p, T, r0, mu0, r, mu, f);
+            for (int c = 0; c < 3; ++c) is[c] *= f[c];
+            is[3] *= f[0];
+        }
+        T_lookup(p, T, r, mu, ext);
+        double m[3];
+        get_mie(p, is, m);
+        const double pr = phaseR(nu), pm = phaseM(p, nu);
+        for (int c = 0; c < 3; ++c) rad[c] = (is[c] * pr + m[c] * pm) * kSunIntensity;
+    } else {
+        for (int c = 0; c < 3; ++c) {
+            rad[c] = 0.0;
+            ext[c] = 1.0;
+        }
+    }
+}
+
+/* :425-533 inScattering (aerial perspective between camera and point) */
+void oracle_in_scattering(const atmo_params *a, const double *T, const double *S,
+                          const double camera[3], const double point_in[3],
+                          const double sundir[3], double shaft_width, double meters_per_unit,
+                          double rad[3], double ext[3]) {
+    P p = mk(a, nullptr);
+    double cam[3] = {camera[0], camera[1], camera[2]};
+    double point[3] = {point_in[0], point_in[1], point_in[2]};
+    double vd[3] = {point[0] - cam[0], point[1] - cam[1], point[2] - cam[2]};
+    double d = std::sqrt(dot3(vd, vd));
+    for (int c = 0; c < 3; ++c) vd[c] /= d;
+    double r = std::sqrt(dot3(cam, cam));
+    if (r < 0.9 * p.Rg) { /* flat-terrain coordinates */
+        cam[2] += p.Rg;
+        point[2] += p.Rg;
+        r = std::sqrt(dot3(cam, cam));
+    }
+    double rMu = dot3(cam, vd);
+    double mu = rMu / r;
+    const double sw = clampd(shaft_width, 0.0, d);
+    for (int c = 0; c < 3; ++c) point[c] -= vd[c] * sw;
+    const double q = rMu * rMu - r * r + p.Rt * p.Rt;
+    const double din = q >= 0.0 ? std::max(-rMu - std::sqrt(q), 0.0) : 0.0; /* C1 */
+    if (din > 0.0 && din < d) {
+        for (int c = 0; c < 3; ++c) cam[c] += din * vd[c];
+        rMu += din;
+        mu = rMu / p.Rt;
+        r = p.Rt;
+        d -= din;
+    }
+    if (r <= p.Rt) {
+        const double nu = dot3(vd, sundir);
+        const double muS = dot3(cam, sundir) / r;
+        const double near = p.Rg + 600.0 / meters_per_unit;
+        if (r < near) {
+            const double f = near / r;
+            r *= f;
+            rMu *= f;
+            for (int c = 0; c < 3; ++c) point[c] *= f;
+        }
+        double r1 = std::sqrt(dot3(point, point));
+        const double rMu1 = dot3(point, vd);
+        double mu1 = rMu1 / r1;
+        const double muS1 = dot3(point, sundir) / r1;
+        if (mu > 0.0) {
+            double x[3], y[3];
+            T_lookup(p, T, r, mu, x);
+            T_lookup(p, T, r1, mu1, y);
+            for (int c = 0; c < 3; ++c) ext[c] = std::min(x[c] / y[c], 1.0);
+        } else {
+            double x[3], y[3];
+            T_lookup(p, T, r1, -mu1, x);
+            T_lookup(p, T, r, -mu, y);
+            for (int c = 0; c < 3; ++c) ext[c] = std::min(x[c] / y[c], 1.0);
+        }
+        auto diff = [&](double m, double rr1, double mm1, double out[4]) {
+            double s0[4], s1[4];
+            tex4D(p, S, 4, r, m, muS, nu, s0);
+            tex4D(p, S, 4, rr1, mm1, muS1, nu, s1);
+            const double e[4] = {ext[0], ext[1], ext[2], ext[0]};
+            for (int c = 0; c < 4; ++c) out[c] = std::max(s0[c] - s1[c] * e[c], 0.0);
+        };
+        double is[4];
+        const double EPS = 0.004; /* HORIZON_HACK, :486-510 */
+        const double lim = -std::sqrt(1.0 - (p.Rg / r) * (p.Rg / r));
+        if (std::fabs(mu - lim) < EPS) {
+            const double w = ((mu - lim) + EPS) / (2.0 * EPS);
+            double A[4], B[4];
+            double m = lim - EPS;
+            r1 = std::sqrt(r * r + d * d + 2.0 * r * d * m);
+            mu1 = (r * m + d) / r1;
+            diff(m, r1, mu1, A);
+            m = lim + EPS;
+            r1 = std::sqrt(r * r + d * d + 2.0 * r * d * m);
+            mu1 = (r * m + d) / r1;
+            diff(m, r1, mu1, B);
+            for (int c = 0; c < 4; ++c) is[c] = A[c] * (1.0 - w) + B[c] * w;
+        } else {
+            diff(mu, r1, mu1, is);
+        }
+        /* :519 smoothstep(0.00, 0.02, muS) */
+        const double t = clampd(muS / 0.02, 0.0, 1.0);
+        is[3] *= t * t * (3.0 - 2.0 * t);
+        double m[3];
+        get_mie(p, is, m);
+        const double pr = phaseR(nu), pm = phaseM(p, nu);
+        for (int c = 0; c < 3; ++c) rad[c] = (is[c] * pr + m[c] * pm) * kSunIntensity;
+    } else {
+        for (int c = 0; c < 3; ++c) {
+            rad[c] = 0.0;
+            ext[c] = 1.0;
+        }
+    }
+}
+
+/* :342-350 sunRadiance (transmittanceWithShadow, :263-265) and :355-361
+ * skyIrradiance, for (r, muS) pairs */
+void oracle_sun_sky_irradiance(const atmo_params *a, const double *T, const double *E, double r,
+                               double muS, double sun[3], double sky[3]) {
+    P p = mk(a, nullptr);
+    if (muS < -std::sqrt(1.0 - (p.Rg / r) * (p.Rg / r))) {
+        sun[0] = sun[1] = sun[2] = 0.0;
+    } else {
+        T_lookup(p, T, r, muS, sun);
+        for (int c = 0; c < 3; ++c) sun[c] *= kSunIntensity;
+    }
+    E_lookup(p, E, r, muS, sky);
+    for (int c = 0; c < 3; ++c) sky[c] *= kSunIntensity;
+}
+
 }  // extern "C"

@@ -80,6 +80,20 @@ ORACLE_API void oracle_pass_copy_irradiance(const atmo_params *p, double k, cons
 ORACLE_API void oracle_precompute(const atmo_params *p, const atmo_options *o, double *T,
                                   double *E, double *S);
 
+/* --- table consumer (src/atmo/examples/atmo/atmosphereShader.glsl) ---
+ * T [TH][TW][3], S [RES_R][RES_MU][RES_NU*RES_MU_S][4], E [SH][SW][3];
+ * positions in the parameters' length unit, planet centre at the origin. */
+ORACLE_API void oracle_sky_radiance(const atmo_params *p, const double *T, const double *S,
+                                    const double camera[3], const double viewdir[3],
+                                    const double sundir[3], double shaft_width, double rad[3],
+                                    double ext[3]);
+ORACLE_API void oracle_in_scattering(const atmo_params *p, const double *T, const double *S,
+                                     const double camera[3], const double point[3],
+                                     const double sundir[3], double shaft_width,
+                                     double meters_per_unit, double rad[3], double ext[3]);
+ORACLE_API void oracle_sun_sky_irradiance(const atmo_params *p, const double *T, const double *E,
+                                          double r, double muS, double sun[3], double sky[3]);
+
 /* Number of OpenMP threads the passes use (0 = library default). */
 ORACLE_API void oracle_set_threads(int n);
 ORACLE_API int oracle_get_threads(void);

@@ -63,6 +63,11 @@ def lib():
             "oracle_pass_copy_inscatterN": (None, [_pp, _dp, _dp]),
             "oracle_pass_copy_irradiance": (None, [_pp, ctypes.c_double, _dp, _dp]),
             "oracle_precompute": (None, [_pp, _po, _dp, _dp, _dp]),
+            "oracle_sky_radiance": (None, [_pp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp]),
+            "oracle_in_scattering": (None, [_pp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double,
+                                            ctypes.c_double, _dp, _dp]),
+            "oracle_sun_sky_irradiance": (None, [_pp, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                                 _dp, _dp]),
             "oracle_set_threads": (None, [ctypes.c_int]),
             "oracle_get_threads": (ctypes.c_int, []),
         }
@@ -278,3 +283,54 @@ def transmittance_segment(p, T, r, mu, d):
 def optical_depth(p, H, r, mu, o=None):
     o = o or options()
     return lib().oracle_optical_depth(p, o, H, r, mu)
+
+
+# ---------------------------------------------------------------- table consumer
+def _v3(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(3))
+
+
+def _tabs(*t):
+    return [np.ascontiguousarray(np.asarray(a, dtype=np.float64)) for a in t]
+
+
+def sky_radiance(p, T, S, camera, dirs, sun, shaft_width=0.0):
+    """atmosphereShader.glsl:371-418 skyRadiance for each row of dirs (n, 3)
+    -> (radiance (n, 3), extinction (n, 3))."""
+    T, S = _tabs(T, S)
+    dirs = np.asarray(dirs, dtype=np.float64).reshape(-1, 3)
+    rad, ext = np.zeros((len(dirs), 3)), np.zeros((len(dirs), 3))
+    cam, sn = _v3(camera), _v3(sun)
+    for i, d in enumerate(dirs):
+        ro, eo = np.zeros(3), np.zeros(3)
+        lib().oracle_sky_radiance(p, _d(T), _d(S), _d(cam), _d(_v3(d)), _d(sn), shaft_width,
+                                  _d(ro), _d(eo))
+        rad[i], ext[i] = ro, eo
+    return rad, ext
+
+
+def in_scattering(p, T, S, camera, points, sun, shaft_width=0.0, meters_per_unit=1000.0):
+    """atmosphereShader.glsl:425-533 inScattering from camera to each row of
+    points (n, 3) -> (radiance (n, 3), extinction (n, 3))."""
+    T, S = _tabs(T, S)
+    points = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+    rad, ext = np.zeros((len(points), 3)), np.zeros((len(points), 3))
+    cam, sn = _v3(camera), _v3(sun)
+    for i, q in enumerate(points):
+        ro, eo = np.zeros(3), np.zeros(3)
+        lib().oracle_in_scattering(p, _d(T), _d(S), _d(cam), _d(_v3(q)), _d(sn), shaft_width,
+                                   meters_per_unit, _d(ro), _d(eo))
+        rad[i], ext[i] = ro, eo
+    return rad, ext
+
+
+def sun_sky_irradiance(p, T, E, rmus):
+    """sunRadiance (:342-350) and skyIrradiance (:355-361) for rows (r, muS)."""
+    T, E = _tabs(T, E)
+    rmus = np.asarray(rmus, dtype=np.float64).reshape(-1, 2)
+    sun, sky = np.zeros((len(rmus), 3)), np.zeros((len(rmus), 3))
+    for i, (r, m) in enumerate(rmus):
+        a, b = np.zeros(3), np.zeros(3)
+        lib().oracle_sun_sky_irradiance(p, _d(T), _d(E), r, m, _d(a), _d(b))
+        sun[i], sky[i] = a, b
+    return sun, sky

@@ -31,6 +31,11 @@ def test_library_exports_every_declared_symbol():
     for n in snames:
         assert hasattr(lib, n), n
     assert SG.param_count() == 201476
+    from prolandatmo import consumer as C
+    cnames = declared("atmo_consumer.h")
+    assert cnames == set(C.SIGNATURES) and len(cnames) == 3
+    for n in cnames:
+        assert hasattr(lib, n), n
     syms = os.popen("nm -D --defined-only %s" % L.LIB_PATH).read()
     for mangled in ("_ZN7proland14preprocessAtmoERKNS_14AtmoParametersEPKc",
                     "_ZN7proland14AtmoParametersC1Ev", "_ZN7proland20preprocessAtmoStatusEv"):
@@ -103,3 +108,24 @@ def test_engine_create_reports_missing_device():
     h = ctypes.c_void_p()
     st = L.lib().atmo_engine_create(ctypes.byref(h), 0)
     assert st == 2 and L.lib().atmo_last_error()
+
+
+def test_consumer_validates_arguments_before_any_device_call():
+    from prolandatmo import consumer as C
+    lib = C.lib()
+    v = C.view([0, 0, 6361.0], [0, 0, 1])
+    p = earth()
+    # n = 0 is a no-op that needs no device
+    assert lib.atmo_sky_radiance(p, None, None, v, None, 0, None, None, None) == 0
+    assert lib.atmo_sun_sky_irradiance(p, None, None, None, 0, None, None, None) == 0
+    # null tables with work to do, a missing view, bad parameters, bad units
+    assert lib.atmo_sky_radiance(p, None, None, v, None, 4, None, None, None) == 1
+    assert lib.atmo_in_scattering(p, None, None, None, None, 4, None, None, None) == 1
+    bad = earth()
+    bad.betaR.y = 0.0
+    assert lib.atmo_sky_radiance(bad, None, None, v, None, 0, None, None, None) == 1
+    assert b"betaR" in L.lib().atmo_last_error()
+    v0 = C.view([0, 0, 6361.0], [0, 0, 1], meters_per_unit=0.0)
+    assert lib.atmo_in_scattering(p, None, None, v0, None, 0, None, None, None) == 1
+    vc = C.view([0, 0, 0], [0, 0, 1])
+    assert lib.atmo_sky_radiance(p, None, None, vc, None, 0, None, None, None) == 1
