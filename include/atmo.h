@@ -81,7 +81,10 @@ typedef struct atmo_options {
     int texels_per_thread;      /* k_inscatterS variant 2, 4 or 8 (bit-identical results);
                                    0 = 4 for the pass-level entry points, autotuned in
                                    atmo_engine_precompute */
-    int reserved[9];
+    int inscatterN_form;        /* k_inscatterN form: 1 direct gathers, 2 LDS-staged rows;
+                                   0 = direct for the pass-level entry points, autotuned
+                                   in atmo_engine_precompute */
+    int reserved[8];
 } atmo_options;
 
 #define ATMO_FLAG_CHECK_NUMERICS 1  /* validate outputs (finite, T in [0,1], E,S >= 0) */
@@ -158,18 +161,22 @@ ATMO_API atmo_status atmo_engine_pass_times(atmo_engine *e, atmo_pass_time *out,
 
 /* Kernel variants the autotuner picked for the engine's current plan.  After
  * the first run of a batch shape not seen before in the process, the engine
- * times every variant of the two dominant passes and keeps the fastest (all
- * variants give bit-identical tables).  Times in ms: inscatterS_ms[i] for 2 / 4 / 8
- * texels per thread (one order-2 + (orders-2) later launches), inscatterN_ms[2c+x]
- * for at most 1 / 2 / 4 mu rows per workgroup (c) without / with XCD-banded
- * tile order (x). */
+ * times every variant of the two dominant passes and keeps the fastest.
+ * k_inscatterS variants (2 / 4 / 8 texels per thread) give bit-identical
+ * tables; the two k_inscatterN forms (direct gathers / LDS-staged blended
+ * rows) compute the same trilinear filter in a different FP32 order.  Times in
+ * ms: inscatterS_ms[i] for 2 / 4 / 8 texels per thread (one order-2 +
+ * (orders-2) later launches), inscatterN_ms[6f + 2c + x] for form f (direct,
+ * staged), at most 1 / 2 / 4 mu rows per workgroup (c), without / with
+ * XCD-banded tile order (x); -1 = does not fit the device's LDS. */
 typedef struct atmo_tuning {
     int tuned;
     int inscatterS_texels_per_thread;
     int inscatterN_rows_per_workgroup;
     int inscatterN_xcd_order;
     float inscatterS_ms[3];
-    float inscatterN_ms[6];
+    float inscatterN_ms[12];
+    int inscatterN_staged_samples;  /* 0: direct gathers; > 0: staged, samples per LDS round */
 } atmo_tuning;
 ATMO_API atmo_status atmo_engine_tuning(atmo_engine *e, atmo_tuning *out);
 

@@ -105,6 +105,8 @@ atmo_status validate_opts(const atmo_options &o) {
     if (o.texels_per_thread != 0 && o.texels_per_thread != 2 && o.texels_per_thread != 4 &&
         o.texels_per_thread != 8)
         return fail(ATMO_ERR_INVALID_ARGUMENT, "texels_per_thread must be 0, 2, 4 or 8");
+    if (o.inscatterN_form < 0 || o.inscatterN_form > 2)
+        return fail(ATMO_ERR_INVALID_ARGUMENT, "inscatterN_form must be 0, 1 or 2");
     return ATMO_OK;
 }
 
@@ -179,7 +181,8 @@ bool same_params(const atmo_params &a, const atmo_params &b) { return memcmp(&a,
 bool same_opts(const atmo_options &a, const atmo_options &b) {
     return a.orders == b.orders && a.transmittance_samples == b.transmittance_samples &&
            a.inscatter_samples == b.inscatter_samples && a.irradiance_samples == b.irradiance_samples &&
-           a.spherical_samples == b.spherical_samples && a.texels_per_thread == b.texels_per_thread;
+           a.spherical_samples == b.spherical_samples && a.texels_per_thread == b.texels_per_thread &&
+           a.inscatterN_form == b.inscatterN_form;
 }
 
 struct DevBuf {
@@ -201,10 +204,25 @@ struct DevBuf {
 
 // Kernel variants chosen by the autotuner, with the times measured for each.
 struct Tuning {
-    int jG = 4, snCap = 4, snXcd = 1;
+    int jG = 4, snCap = 4, snXcd = 1, snSb = 0;
     float msJ[3] = {0, 0, 0};     // G = 2, 4, 8 (order-2 + order->=3 launches)
-    float msN[6] = {0, 0, 0, 0, 0, 0};  // caps 1, 2, 4 x (plain, XCD order)
+    float msN[12] = {};           // [direct, staged 2][caps 1, 2, 4][plain, XCD order]
 };
+
+// static LDS of k_inscatterN (sample staging), on top of its dynamic row blend
+constexpr size_t kSnStaticLds = 16400;
+// samples per LDS round of the staged k_inscatterN form
+constexpr int kSnStagedSamples = 2;
+// largest rows-per-workgroup x RNU*RMU_S of a batch (k_inscatterN staging)
+int sn_max_row_texels(const std::vector<atmo_params> &ps, int snCap) {
+    int m = 0;
+    for (const atmo_params &p : ps) {
+        const int W4 = p.RES_MU_S * p.RES_NU;
+        const int rpw = std::max(1, std::min(std::min(snCap, 4), 256 / W4));
+        m = std::max(m, rpw * W4);
+    }
+    return m;
+}
 
 struct Plan {
     std::vector<atmo_params> params;
@@ -220,6 +238,7 @@ struct Plan {
     int jG = 4;        // k_inscatterS texels per thread
     int snCap = 4;     // k_inscatterN max mu rows per workgroup
     int snXcd = 1;     // k_inscatterN XCD-banded tile order
+    int snSb = 0;      // k_inscatterN: 0 direct gathers, > 0 LDS-staged rows (samples per round)
     bool tuned = false;
     void drop_graph() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -428,7 +447,8 @@ atmo_status launch_all(Plan &pl, hipStream_t st, PassTimer *tm = nullptr) {
                                   dirs + pl.dirE_off, first, first ? pl.ldsE1 : pl.ldsEN, st));
         HIPCHK(t.end());
         HIPCHK(t.begin(5, order));
-        HIPCHK(launch_inscatterN(S, T + pl.off[P_SN], pl.cnt[P_SN], o.inscatter_samples, pl.snCap, st));
+        HIPCHK(launch_inscatterN(S, T + pl.off[P_SN], pl.cnt[P_SN], o.inscatter_samples, pl.snCap, pl.snSb,
+                                 lds_inscatterN(sn_max_row_texels(pl.params, pl.snCap), pl.snSb), st));
         HIPCHK(t.end());
     }
     return ATMO_OK;
@@ -529,10 +549,19 @@ atmo_status autotune(Plan &pl, hipStream_t st, Tuning &tu) {
     DevBuf ds, dt;
     HIPCHK(ds.ensure(hs.size() * sizeof(SetDesc)));
     HIPCHK(hipMemcpy(ds.p, hs.data(), hs.size() * sizeof(SetDesc), hipMemcpyHostToDevice));
-    const int caps[3] = {1, 2, 4};
+    const int caps[3] = {1, 2, 4}, sbs[2] = {0, kSnStagedSamples};
+    int dev = 0, maxLds = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     float bestN = 1e30f;
+    for (int si = 0; si < 2; ++si)
     for (int ci = 0; ci < 3; ++ci)
         for (int xo = 0; xo < 2; ++xo) {
+            const size_t lds = lds_inscatterN(sn_max_row_texels(pl.params, caps[ci]), sbs[si]);
+            if (lds + kSnStaticLds > (size_t)maxLds) {  // does not fit this device's LDS
+                tu.msN[si * 6 + ci * 2 + xo] = -1.f;
+                continue;
+            }
             std::vector<uint2> tiles;
             int off[P_COUNT], cnt[P_COUNT];
             build_tiles(pl.params, o, tiles, off, cnt, caps[ci], xo);
@@ -541,14 +570,15 @@ atmo_status autotune(Plan &pl, hipStream_t st, Tuning &tu) {
             float ms = 0.f;
             atmo_status rc = time_launch(st, ms, [&]() {
                 return launch_inscatterN((const SetDesc *)ds.p, (const uint2 *)dt.p + off[P_SN], cnt[P_SN],
-                                         o.inscatter_samples, caps[ci], st);
+                                         o.inscatter_samples, caps[ci], sbs[si], lds, st);
             });
             if (rc != ATMO_OK) return rc;
-            tu.msN[ci * 2 + xo] = ms;
+            tu.msN[si * 6 + ci * 2 + xo] = ms;
             if (ms < bestN) {
                 bestN = ms;
                 tu.snCap = caps[ci];
                 tu.snXcd = xo;
+                tu.snSb = sbs[si];
             }
         }
     return ATMO_OK;
@@ -561,6 +591,7 @@ atmo_status apply_tuning(Plan &pl, const Tuning &tu) {
     pl.jG = pl.opts.texels_per_thread ? pl.opts.texels_per_thread : tu.jG;
     pl.snCap = tu.snCap;
     pl.snXcd = tu.snXcd;
+    pl.snSb = pl.opts.inscatterN_form ? (pl.opts.inscatterN_form == 2 ? kSnStagedSamples : 0) : tu.snSb;
     pl.tuned = true;
     if (retile) {
         std::vector<uint2> tiles;
@@ -578,6 +609,7 @@ atmo_status build_plan(atmo_engine *e, const atmo_params *sets, int nsets, const
     pl->params.assign(sets, sets + nsets);
     pl->opts = o;
     pl->jG = o.texels_per_thread ? o.texels_per_thread : 4;
+    pl->snSb = o.inscatterN_form == 2 ? kSnStagedSamples : 0;
     // work arena
     size_t total = 0;
     std::vector<size_t> base(nsets);
@@ -724,7 +756,8 @@ atmo_status atmo_engine_tuning(atmo_engine *e, atmo_tuning *out) {
     out->inscatterN_rows_per_workgroup = e->last_tuning.snCap;
     out->inscatterN_xcd_order = e->last_tuning.snXcd;
     for (int i = 0; i < 3; ++i) out->inscatterS_ms[i] = e->last_tuning.msJ[i];
-    for (int i = 0; i < 6; ++i) out->inscatterN_ms[i] = e->last_tuning.msN[i];
+    for (int i = 0; i < 12; ++i) out->inscatterN_ms[i] = e->last_tuning.msN[i];
+    out->inscatterN_staged_samples = e->last_tuning.snSb;
     return ATMO_OK;
 }
 
@@ -924,7 +957,12 @@ static atmo_status pass_run(atmo_engine *e, const atmo_params *p, const atmo_opt
                                   first, lds, hs));
         break;
     }
-    case P_SN: HIPCHK(launch_inscatterN(S, T + off[P_SN], cnt[P_SN], o.inscatter_samples, 4, hs)); break;
+    case P_SN: {
+        const int sb = o.inscatterN_form == 2 ? kSnStagedSamples : 0;
+        const size_t lds = lds_inscatterN(sn_max_row_texels(std::vector<atmo_params>(1, *p), 4), sb);
+        HIPCHK(launch_inscatterN(S, T + off[P_SN], cnt[P_SN], o.inscatter_samples, 4, sb, lds, hs));
+        break;
+    }
     }
     HIPCHK(hipStreamSynchronize(hs));
     g_err.clear();

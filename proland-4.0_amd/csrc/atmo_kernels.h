@@ -31,9 +31,12 @@ hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles
 hipError_t launch_irradianceN(const SetDesc *sets, const uint2 *tiles, int ntiles, int NI,
                               const void *dirs, bool first, size_t lds, hipStream_t st);
 // rpw_cap: at most this many mu rows per workgroup (1, 2 or 4); the tile table
-// must have been built with the same cap (sn_tiles_per_set)
+// must have been built with the same cap.  sb = 0: direct gathers; sb > 0:
+// LDS-staged blended rows, sb samples per round, lds = lds_inscatterN(max over
+// the batch of rows-per-workgroup x RNU*RMU_S, sb) bytes of dynamic LDS.
+size_t lds_inscatterN(int max_row_texels, int sb);
 hipError_t launch_inscatterN(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
-                             int rpw_cap, hipStream_t st);
+                             int rpw_cap, int sb, size_t lds, hipStream_t st);
 hipError_t launch_check(const SetDesc *sets, const uint2 *tiles, int ntiles, unsigned int *flags,
                         hipStream_t st);
 

@@ -592,11 +592,21 @@ struct SnEntry {
     float4 ir;    // 1 / r_i
 };
 
+// STAGED = false: each texel gathers its 8 taps per sample from dJ directly
+// (global loads).  STAGED = true: per sample the workgroup first blends the
+// 4 (r, mu) table rows of every row it covers into one row
+// B_i[x] = sum_q w_q dJ[row_q][x] in LDS (coalesced loads, 4 per texel
+// instead of 8, shared by all texels that hit x), then each texel reads two
+// neighbours of B_i and lerps in x.  Both compute the same trilinear filter;
+// they differ only in FP32 rounding order.  `sb` samples are blended per LDS
+// round (dynamic LDS: sb * rows * RNU*RMU_S float4).
+template <bool STAGED>
 __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ sets,
                                                     const uint2 *__restrict__ tiles,
-                                                    int nsamp, int rpw_cap) {
+                                                    int nsamp, int rpw_cap, int sb) {
     __shared__ SnEntry ent[kNRows][kNChunk];
     __shared__ float rowdx[kNRows];
+    extern __shared__ float4 blend[];   // [sb][nrows * W4] (STAGED)
     const uint2 tl = tiles[blockIdx.x];
     const SetDesc &s = sets[tl.x];
     const int RPW = sn_rows_per_wg(s.W4, rpw_cap);
@@ -639,6 +649,10 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
         }
     }
 
+    typedef __attribute__((address_space(1))) const char gchar;
+    gchar *J = (gchar *)s.dJ4;
+    const int W4m1 = s.W4 - 1;
+    const unsigned W4m2 = (unsigned)(s.W4 - 2);
     for (int c0 = 0; c0 <= nsamp; c0 += kNChunk) {
         const int nc = min(kNChunk, nsamp + 1 - c0);
         __syncthreads();
@@ -693,61 +707,121 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
             ent[row][js] = en;
         }
         __syncthreads();
-        typedef __attribute__((address_space(1))) const char gchar;
-        gchar *J = (gchar *)s.dJ4;
-        const int W4m1 = s.W4 - 1;
-        const unsigned W4m2 = (unsigned)(s.W4 - 2);
+        if (!STAGED) {
 #pragma unroll
-        for (int j = 0; j < MAXT; ++j) {
-            if (j >= nmine) break;
-            const int e = threadIdx.x + j * (int)blockDim.x;
-            if (e >= per) break;
-            const SnEntry *er = ent[rowi[j]];
-            const float kbase = (float)(k0[j] * s.RMUS);
-            float a0 = acc[j][0], a1 = acc[j][1], a2 = acc[j][2];
-            for (int js = 0; js < nc; ++js) {
-                const SnEntry en = er[js];
-                const float muSi = fmaf(nuf[j], en.tw.w, muSr[j]) * en.ir.x;
-                // x: texel column; clamped to [0, W4-1] and split so that i0 + 1 <= W4 - 1
-                float tx = __builtin_amdgcn_fmed3f(kbase + tex4d_txs_f(s, muSi), 0.0f, (float)W4m1);
-                const unsigned i0 = min(__float2uint_rz(tx), W4m2);
-                const float ax = tx - (float)i0;
-                float3 v = make_float3(0.f, 0.f, 0.f);
-                const float wr[4] = {en.w.x, en.w.y, en.w.z, en.w.w};
-                const unsigned br[4] = {(unsigned)en.base.x, (unsigned)en.base.y, (unsigned)en.base.z,
-                                        (unsigned)en.base.w};
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const unsigned off = br[q] + i0 * 16u;
-                    const float3 c0v = ld3((const gfloat4 *)(J + off));
-                    const float3 c1v = ld3((const gfloat4 *)(J + off + 16u));
-                    const float wq = wr[q], wa = wr[q] * ax;
-                    v.x = fmaf(wq, c0v.x, fmaf(wa, c1v.x - c0v.x, v.x));
-                    v.y = fmaf(wq, c0v.y, fmaf(wa, c1v.y - c0v.y, v.y));
-                    v.z = fmaf(wq, c0v.z, fmaf(wa, c1v.z - c0v.z, v.z));
-                }
-                if (fk[j] != 0.0f) {  // off-grid nu: blend with the next slice
-                    const float tx2 = __builtin_amdgcn_fmed3f(tx + (float)s.RMUS, 0.0f, (float)W4m1);
-                    const unsigned j0 = min(__float2uint_rz(tx2), W4m2);
-                    const float bx = tx2 - (float)j0;
-                    float3 v2 = make_float3(0.f, 0.f, 0.f);
+            for (int j = 0; j < MAXT; ++j) {
+                if (j >= nmine) break;
+                const int e = threadIdx.x + j * (int)blockDim.x;
+                if (e >= per) break;
+                const SnEntry *er = ent[rowi[j]];
+                const float kbase = (float)(k0[j] * s.RMUS);
+                float a0 = acc[j][0], a1 = acc[j][1], a2 = acc[j][2];
+                for (int js = 0; js < nc; ++js) {
+                    const SnEntry en = er[js];
+                    const float muSi = fmaf(nuf[j], en.tw.w, muSr[j]) * en.ir.x;
+                    // x: texel column; clamped to [0, W4-1] and split so that i0 + 1 <= W4 - 1
+                    float tx = __builtin_amdgcn_fmed3f(kbase + tex4d_txs_f(s, muSi), 0.0f, (float)W4m1);
+                    const unsigned i0 = min(__float2uint_rz(tx), W4m2);
+                    const float ax = tx - (float)i0;
+                    float3 v = make_float3(0.f, 0.f, 0.f);
+                    const float wr[4] = {en.w.x, en.w.y, en.w.z, en.w.w};
+                    const unsigned br[4] = {(unsigned)en.base.x, (unsigned)en.base.y, (unsigned)en.base.z,
+                                            (unsigned)en.base.w};
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        const unsigned off = br[q] + j0 * 16u;
+                        const unsigned off = br[q] + i0 * 16u;
                         const float3 c0v = ld3((const gfloat4 *)(J + off));
                         const float3 c1v = ld3((const gfloat4 *)(J + off + 16u));
-                        const float wq = wr[q], wa = wr[q] * bx;
-                        v2.x = fmaf(wq, c0v.x, fmaf(wa, c1v.x - c0v.x, v2.x));
-                        v2.y = fmaf(wq, c0v.y, fmaf(wa, c1v.y - c0v.y, v2.y));
-                        v2.z = fmaf(wq, c0v.z, fmaf(wa, c1v.z - c0v.z, v2.z));
+                        const float wq = wr[q], wa = wr[q] * ax;
+                        v.x = fmaf(wq, c0v.x, fmaf(wa, c1v.x - c0v.x, v.x));
+                        v.y = fmaf(wq, c0v.y, fmaf(wa, c1v.y - c0v.y, v.y));
+                        v.z = fmaf(wq, c0v.z, fmaf(wa, c1v.z - c0v.z, v.z));
                     }
-                    v = lerp3(v, v2, fk[j]);
+                    if (fk[j] != 0.0f) {  // off-grid nu: blend with the next slice
+                        const float tx2 = __builtin_amdgcn_fmed3f(tx + (float)s.RMUS, 0.0f, (float)W4m1);
+                        const unsigned j0 = min(__float2uint_rz(tx2), W4m2);
+                        const float bx = tx2 - (float)j0;
+                        float3 v2 = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const unsigned off = br[q] + j0 * 16u;
+                            const float3 c0v = ld3((const gfloat4 *)(J + off));
+                            const float3 c1v = ld3((const gfloat4 *)(J + off + 16u));
+                            const float wq = wr[q], wa = wr[q] * bx;
+                            v2.x = fmaf(wq, c0v.x, fmaf(wa, c1v.x - c0v.x, v2.x));
+                            v2.y = fmaf(wq, c0v.y, fmaf(wa, c1v.y - c0v.y, v2.y));
+                            v2.z = fmaf(wq, c0v.z, fmaf(wa, c1v.z - c0v.z, v2.z));
+                        }
+                        v = lerp3(v, v2, fk[j]);
+                    }
+                    a0 = fmaf(v.x, en.tw.x, a0);
+                    a1 = fmaf(v.y, en.tw.y, a1);
+                    a2 = fmaf(v.z, en.tw.z, a2);
                 }
-                a0 = fmaf(v.x, en.tw.x, a0);
-                a1 = fmaf(v.y, en.tw.y, a1);
-                a2 = fmaf(v.z, en.tw.z, a2);
+                acc[j][0] = a0; acc[j][1] = a1; acc[j][2] = a2;
             }
-            acc[j][0] = a0; acc[j][1] = a1; acc[j][2] = a2;
+        } else {
+            for (int s0 = 0; s0 < nc; s0 += sb) {
+                const int ns = min(sb, nc - s0);
+                // (a) blended rows of samples s0 .. s0+ns-1: entry e = (row, x), as the texels
+                for (int k = 0; k < ns; ++k) {
+#pragma unroll
+                    for (int j = 0; j < MAXT; ++j) {
+                        if (j >= nmine) break;
+                        const int e = threadIdx.x + j * (int)blockDim.x;
+                        if (e >= per) break;
+                        const SnEntry &en = ent[rowi[j]][s0 + k];
+                        const unsigned xo = (unsigned)(e - rowi[j] * s.W4) * 16u;
+                        const float3 t0 = ld3((const gfloat4 *)(J + (unsigned)en.base.x + xo));
+                        const float3 t1 = ld3((const gfloat4 *)(J + (unsigned)en.base.y + xo));
+                        const float3 t2 = ld3((const gfloat4 *)(J + (unsigned)en.base.z + xo));
+                        const float3 t3 = ld3((const gfloat4 *)(J + (unsigned)en.base.w + xo));
+                        float4 b;
+                        b.x = fmaf(en.w.w, t3.x, fmaf(en.w.z, t2.x, fmaf(en.w.y, t1.x, en.w.x * t0.x)));
+                        b.y = fmaf(en.w.w, t3.y, fmaf(en.w.z, t2.y, fmaf(en.w.y, t1.y, en.w.x * t0.y)));
+                        b.z = fmaf(en.w.w, t3.z, fmaf(en.w.z, t2.z, fmaf(en.w.y, t1.z, en.w.x * t0.z)));
+                        b.w = 0.f;
+                        blend[(size_t)k * per + e] = b;
+                    }
+                }
+                __syncthreads();
+                // (b) per texel: x lerp of the blended row
+#pragma unroll
+                for (int j = 0; j < MAXT; ++j) {
+                    if (j >= nmine) break;
+                    const int e = threadIdx.x + j * (int)blockDim.x;
+                    if (e >= per) break;
+                    const SnEntry *er = ent[rowi[j]];
+                    const float kbase = (float)(k0[j] * s.RMUS);
+                    const int rowbase = rowi[j] * s.W4;
+                    float a0 = acc[j][0], a1 = acc[j][1], a2 = acc[j][2];
+                    for (int k = 0; k < ns; ++k) {
+                        const SnEntry &en = er[s0 + k];
+                        const float muSi = fmaf(nuf[j], en.tw.w, muSr[j]) * en.ir.x;
+                        float tx = __builtin_amdgcn_fmed3f(kbase + tex4d_txs_f(s, muSi), 0.0f, (float)W4m1);
+                        const unsigned i0 = min(__float2uint_rz(tx), W4m2);
+                        const float ax = tx - (float)i0;
+                        const float4 *B = blend + (size_t)k * per + rowbase;
+                        const float4 p0 = B[i0], p1 = B[i0 + 1];
+                        float3 v = make_float3(fmaf(ax, p1.x - p0.x, p0.x), fmaf(ax, p1.y - p0.y, p0.y),
+                                               fmaf(ax, p1.z - p0.z, p0.z));
+                        if (fk[j] != 0.0f) {  // off-grid nu: blend with the next slice
+                            const float tx2 = __builtin_amdgcn_fmed3f(tx + (float)s.RMUS, 0.0f, (float)W4m1);
+                            const unsigned j0 = min(__float2uint_rz(tx2), W4m2);
+                            const float bx = tx2 - (float)j0;
+                            const float4 q0 = B[j0], q1 = B[j0 + 1];
+                            const float3 v2 = make_float3(fmaf(bx, q1.x - q0.x, q0.x), fmaf(bx, q1.y - q0.y, q0.y),
+                                                          fmaf(bx, q1.z - q0.z, q0.z));
+                            v = lerp3(v, v2, fk[j]);
+                        }
+                        a0 = fmaf(v.x, en.tw.x, a0);
+                        a1 = fmaf(v.y, en.tw.y, a1);
+                        a2 = fmaf(v.z, en.tw.z, a2);
+                    }
+                    acc[j][0] = a0; acc[j][1] = a1; acc[j][2] = a2;
+                }
+                __syncthreads();
+            }
         }
     }
 #pragma unroll
@@ -862,10 +936,16 @@ hipError_t launch_irradianceN(const SetDesc *sets, const uint2 *tiles, int ntile
                            (const DirE *)dirs);
     return hipGetLastError();
 }
+size_t lds_inscatterN(int max_row_texels, int sb) {
+    return sb > 0 ? sizeof(float4) * (size_t)sb * (size_t)max_row_texels : 0;
+}
 hipError_t launch_inscatterN(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
-                             int rpw_cap, hipStream_t st) {
+                             int rpw_cap, int sb, size_t lds, hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_inscatterN, dim3(ntiles), dim3(256), 0, st, sets, tiles, nsamp, rpw_cap);
+    if (sb > 0)
+        hipLaunchKernelGGL(k_inscatterN<true>, dim3(ntiles), dim3(256), lds, st, sets, tiles, nsamp, rpw_cap, sb);
+    else
+        hipLaunchKernelGGL(k_inscatterN<false>, dim3(ntiles), dim3(256), 0, st, sets, tiles, nsamp, rpw_cap, 0);
     return hipGetLastError();
 }
 hipError_t launch_check(const SetDesc *sets, const uint2 *tiles, int ntiles, unsigned int *flags,

@@ -336,7 +336,9 @@ atmo_status atmo_surrogate_build_dataset(atmo_surrogate *s, const atmo_params *s
     HIPCHK(s->Xa.ensure(rows * kXS * 4));
     HIPCHK(s->Ta.ensure(rows * kNOUT * 4));
     atmo_options on = *opts, o1 = *opts;
-    on.flags |= ATMO_FLAG_NO_GRAPH;
+    // every chunk of seeded sets is a new batch shape: no graph capture, no
+    // per-shape autotuning (the default kernel variants)
+    on.flags |= ATMO_FLAG_NO_GRAPH | ATMO_FLAG_NO_AUTOTUNE;
     o1.orders = 1;
     o1.flags |= ATMO_FLAG_NO_GRAPH;
     const int CH = 16;

@@ -33,13 +33,15 @@ PASS_NAMES = ("transmittance", "irradiance1", "inscatter1", "inscatterS", "irrad
 class AtmoTuning(ctypes.Structure):
     _fields_ = [("tuned", ctypes.c_int), ("inscatterS_texels_per_thread", ctypes.c_int),
                 ("inscatterN_rows_per_workgroup", ctypes.c_int), ("inscatterN_xcd_order", ctypes.c_int),
-                ("inscatterS_ms", ctypes.c_float * 3), ("inscatterN_ms", ctypes.c_float * 6)]
+                ("inscatterS_ms", ctypes.c_float * 3), ("inscatterN_ms", ctypes.c_float * 12),
+                ("inscatterN_staged_samples", ctypes.c_int)]
 
     def as_dict(self):
         return {"tuned": self.tuned, "inscatterS_texels_per_thread": self.inscatterS_texels_per_thread,
                 "inscatterN_rows_per_workgroup": self.inscatterN_rows_per_workgroup,
                 "inscatterN_xcd_order": self.inscatterN_xcd_order,
                 "inscatterS_ms": [round(x, 3) for x in self.inscatterS_ms],
+                "inscatterN_staged_samples": self.inscatterN_staged_samples,
                 "inscatterN_ms": [round(x, 3) for x in self.inscatterN_ms]}
 
 

@@ -60,7 +60,8 @@ class AtmoOptions(ctypes.Structure):
         ("orders", ctypes.c_int), ("transmittance_samples", ctypes.c_int),
         ("inscatter_samples", ctypes.c_int), ("irradiance_samples", ctypes.c_int),
         ("spherical_samples", ctypes.c_int), ("flags", ctypes.c_int),
-        ("texels_per_thread", ctypes.c_int), ("reserved", ctypes.c_int * 9),
+        ("texels_per_thread", ctypes.c_int), ("inscatterN_form", ctypes.c_int),
+        ("reserved", ctypes.c_int * 8),
     ]
 
 
@@ -95,11 +96,13 @@ def earth(**over):
 
 
 def options(orders=4, transmittance_samples=500, inscatter_samples=50,
-            irradiance_samples=32, spherical_samples=16, flags=0, texels_per_thread=0):
+            irradiance_samples=32, spherical_samples=16, flags=0, texels_per_thread=0,
+            inscatterN_form=0):
     """Reference sample counts (common.glsl:94-97) and order count (PreprocessAtmo.cpp:351);
-    texels_per_thread pins the k_inscatterS variant (0 = default / autotuned)."""
+    texels_per_thread and inscatterN_form pin kernel variants (0 = default / autotuned)."""
     o = AtmoOptions()
     o.texels_per_thread = texels_per_thread
+    o.inscatterN_form = inscatterN_form
     o.orders = orders
     o.transmittance_samples = transmittance_samples
     o.inscatter_samples = inscatter_samples
