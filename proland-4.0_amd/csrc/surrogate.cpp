@@ -110,6 +110,8 @@ struct atmo_surrogate {
     int device = 0;
     hipStream_t st = nullptr;
     Buf p, g, m, v, wb, wt;   // FP32 params/grads/Adam; bf16 W and W^T of layers 2..4
+    Buf fz;                   // fused-inference weights (prep_fused)
+    Buf iset;                 // InferSet array of the last fused launch
     TargetNorm tn{{0, 0, 0}, {1, 1, 1}, 1e-6f};
     int adam_t = 0;
     Buf Xa, Ta;
@@ -153,9 +155,27 @@ atmo_status ensure_cap(atmo_surrogate *s, int rows, bool train) {
     return ATMO_OK;
 }
 
+// fused-inference weight buffer: a1h, a1l [256][16], a5h, a5l [4][256] (bf16), b234 [3][256] (FP32)
+constexpr size_t FZ_A1H = 0, FZ_A1L = FZ_A1H + kNH * 16 * 2, FZ_A5H = FZ_A1L + kNH * 16 * 2,
+                 FZ_A5L = FZ_A5H + kNOUT * kNH * 2, FZ_B234 = FZ_A5L + kNOUT * kNH * 2,
+                 FZ_BYTES = FZ_B234 + 3 * kNH * 4;
+
+FusedWeights fused_weights(atmo_surrogate *s) {
+    char *z = (char *)s->fz.p;
+    const float *P = s->p.as<float>();
+    return FusedWeights{(const __bf16 *)(z + FZ_A1H), (const __bf16 *)(z + FZ_A1L), s->Wt(2),
+                        (const float *)(z + FZ_B234), (const __bf16 *)(z + FZ_A5H), (const __bf16 *)(z + FZ_A5L),
+                        P + OFF_B5};
+}
+
 atmo_status refresh_bf16(atmo_surrogate *s) {
     for (int L = 2; L <= 4; ++L)
         HIPCHK(to_bf16(s->p.as<float>() + offW(L), s->Wb(L), s->Wt(L), s->st));
+    const float *P = s->p.as<float>();
+    char *z = (char *)s->fz.p;
+    HIPCHK(prep_fused(P + OFF_W1, P + OFF_B1, P + OFF_W5, P + offB(2), P + offB(3), P + offB(4),
+                      (__bf16 *)(z + FZ_A1H), (__bf16 *)(z + FZ_A1L), (__bf16 *)(z + FZ_A5H),
+                      (__bf16 *)(z + FZ_A5L), (float *)(z + FZ_B234), s->st));
     return ATMO_OK;
 }
 
@@ -248,6 +268,7 @@ atmo_status atmo_surrogate_create(atmo_surrogate **out, int device, uint64_t see
     HIPCHK(s->v.ensure(NPARAM * 4));
     HIPCHK(s->wb.ensure((size_t)3 * kNH * kNH * 2));
     HIPCHK(s->wt.ensure((size_t)3 * kNH * kNH * 2));
+    HIPCHK(s->fz.ensure(FZ_BYTES));
     HIPCHK(s->loss.ensure(4));
     // Xavier-uniform weights, zero biases
     std::vector<float> h(NPARAM, 0.0f);
@@ -473,21 +494,23 @@ atmo_status atmo_surrogate_precompute(atmo_surrogate *s, const atmo_params *sets
     HIPCHK(hipEventRecord(ev[0], s->st));
     STCHK(atmo_engine_precompute(s->e1, sets, nsets, &o1, out, s->st));
     HIPCHK(hipEventRecord(ev[1], s->st));
-    // (2) orders >= 2 from the network
-    const int CH = 1 << 20;
-    STCHK(ensure_cap(s, CH, false));
-    for (int i = 0; i < nsets; ++i) {
-        const atmo_params &p = sets[i];
-        const FeatSet fs = feat_set(p);
-        const int total = fs.n4 + fs.SW * fs.SH;
-        for (int r0 = 0; r0 < total; r0 += CH) {
-            const int n = std::min(CH, total - r0);
-            const int np = (n + 127) / 128 * 128;
-            HIPCHK(infer_rows(fs, r0, n, s->X.as<float>(), s->st));
-            STCHK(forward(s, np, false, nullptr, s->Y.as<float>()));
-            HIPCHK(apply(r0, n, fs.n4, s->Y.as<float>(), s->tn, (float4 *)out[i].inscatter, out[i].irradiance,
-                         s->st));
+    // (2) orders >= 2 from the network: one fused launch for the batch
+    // (features, five layers and the combination with the exact part)
+    {
+        std::vector<InferSet> hs(nsets);
+        const int rpb = fused_rows_per_block();
+        int tiles = 0;
+        for (int i = 0; i < nsets; ++i) {
+            hs[i].fs = feat_set(sets[i]);
+            hs[i].tile0 = tiles;
+            hs[i].S = (float4 *)out[i].inscatter;
+            hs[i].E = out[i].irradiance;
+            tiles += (hs[i].fs.n4 + hs[i].fs.SW * hs[i].fs.SH + rpb - 1) / rpb;
         }
+        HIPCHK(s->iset.ensure(hs.size() * sizeof(InferSet)));
+        HIPCHK(hipMemcpyAsync(s->iset.p, hs.data(), hs.size() * sizeof(InferSet), hipMemcpyHostToDevice, s->st));
+        HIPCHK(mlp_infer((const InferSet *)s->iset.p, nsets, tiles, fused_weights(s), s->tn, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));  // hs must outlive the copy
     }
     HIPCHK(hipEventRecord(ev[2], s->st));
     // (3) judge: plain recompute of ceil(fraction * nsets) seeded sets

@@ -30,6 +30,24 @@ struct TargetNorm {
     float mu[3], sd[3], eps;
 };
 
+// Weights of the fused inference kernel (device pointers), from prep_fused.
+struct FusedWeights {
+    const __bf16 *a1h, *a1l;  // [256][16] layer 1 (column 11: bias), bf16 hi / lo
+    const __bf16 *wt;         // [3][256][256] layers 2..4 as [out][in]
+    const float *b234;        // [3][256] biases of layers 2..4
+    const __bf16 *a5h, *a5l;  // [4][256] layer 5 as [out][in], bf16 hi / lo
+    const float *b5;          // [4]
+};
+
+// One set of a fused inference launch: its features, its first workgroup
+// (tiles of fused_rows_per_block() rows: S texels, then E texels) and outputs.
+struct InferSet {
+    FeatSet fs;
+    int tile0, pad[3];
+    float4 *S;
+    float *E;
+};
+
 hipError_t gemm(bool ta, bool tb, int epi, const __bf16 *A, const __bf16 *B, int M, int N, int K,
                 int nsplit, const GemmEpi &ep, hipStream_t st);
 hipError_t layer1_fwd(const float *X, const float *W1, const float *b1, int rows, float *Z1,
@@ -53,6 +71,15 @@ hipError_t code_rows(const FeatSet &fs, const unsigned *codes, int n, float *X, 
 hipError_t norm_targets(float *Tg, size_t n, const TargetNorm &tn, hipStream_t st);
 hipError_t apply(int row0, int n, int n4, const float *Y, const TargetNorm &tn, float4 *S, float *E,
                  hipStream_t st);
+hipError_t prep_fused(const float *W1, const float *b1, const float *W5, const float *b2, const float *b3,
+                      const float *b4, __bf16 *a1h, __bf16 *a1l, __bf16 *a5h, __bf16 *a5l, float *b234,
+                      hipStream_t st);
+// Fused inference of every S and E texel of a batch of sets (device array
+// `sets`, tile0 ascending, ntiles workgroups in all): adds the predicted
+// orders >= 2 to S.rgb and writes E.
+int fused_rows_per_block();
+hipError_t mlp_infer(const InferSet *sets, int nsets, int ntiles, const FusedWeights &fw, const TargetNorm &tn,
+                     hipStream_t st);
 hipError_t gather(const float *Xa, const float *Ta, size_t n, int B, unsigned long long seed,
                   float *X, float *Tg, hipStream_t st);
 hipError_t relerr(const float *a, const float *b, size_t n, int stride, int ncomp, float floorv,

@@ -13,7 +13,7 @@ namespace sg {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
-__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float sigm(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 __device__ __forceinline__ float silu(float z) { return z * sigm(z); }
 __device__ __forceinline__ float dsilu(float z) {
     const float s = sigm(z);
@@ -465,6 +465,280 @@ __global__ __launch_bounds__(256) void k_absmax(const float *__restrict__ a, siz
 }
 
 // ---------------------------------------------------------------------------
+// Fused inference: texel features -> layer 1 -> 3 x (256x256 + bias + SiLU)
+// -> layer 5 -> orders >= 2 added to S / written to E, all in one kernel with
+// the activations in registers.
+//
+// The products are computed transposed, D = W^T H^T (output features x rows),
+// with v_mfma_f32_32x32x16_bf16: the A operand is a 32-feature x 16-k slice of
+// the weight (W^T rows, staged in LDS), the B operand a 16-k x 32-row slice of
+// the activations, held by each lane for its own row (lane l: row l%32,
+// features 16kb + 8(l/32) .. +7 of every k block kb).  A D tile gives lane l
+// the features 8q + 4(l/32) + t (q, t < 4) of row l%32; one swap of 4 values
+// with lane l^32 turns it into the B layout of the next layer.  Each wave owns
+// 32*RBW rows through all five layers; a workgroup stages one 256x256 weight
+// (rows padded to 528 B, conflict-free 16-byte LDS reads) per layer.
+// Layers 1 and 5 use the same MFMA with FP32 weights split into bf16 hi + lo
+// parts (x W ~ xh Wh + xh Wl + xl Wh for layer 1, whose bias rides in input
+// column 11; H4 is bf16 already, so layer 5 needs Wh + Wl only): relative
+// error ~1e-5, well inside the bf16 rounding of the next activation.
+// ---------------------------------------------------------------------------
+constexpr int kRowPad = 264;   // bf16 per staged weight row (512 B + 16 B pad)
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned pack_bf16(float a, float b) {
+    const unsigned lo = __builtin_bit_cast(unsigned short, (__bf16)a);
+    const unsigned hi = __builtin_bit_cast(unsigned short, (__bf16)b);
+    return lo | (hi << 16);
+}
+
+// D tile (features x 32 rows) -> SiLU(D + bias) as the B operands of k blocks
+// 2nt (lo) and 2nt + 1 (hi).  bias: the tile's 32 biases or nullptr.
+__device__ __forceinline__ void tile_to_b(const f32x16 &acc, const float *bias, int h, bf16x8 &lo,
+                                          bf16x8 &hi) {
+    unsigned P[4][2];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float a[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const float z = acc[4 * q + t] + (bias ? bias[8 * q + 4 * h + t] : 0.0f);
+            a[t] = silu(z);
+        }
+        P[q][0] = pack_bf16(a[0], a[1]);
+        P[q][1] = pack_bf16(a[2], a[3]);
+    }
+    // lane h = 0 keeps features 0-3, 16-19 and needs 4-7, 20-23 from lane l^32;
+    // lane h = 1 keeps 12-15, 28-31 and needs 8-11, 24-27
+    unsigned ra[2], rb[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        ra[i] = (unsigned)__shfl_xor((int)(h ? P[0][i] : P[1][i]), 32, 64);
+        rb[i] = (unsigned)__shfl_xor((int)(h ? P[2][i] : P[3][i]), 32, 64);
+    }
+    u32x4 L, Hh;
+    if (h == 0) {
+        L = u32x4{P[0][0], P[0][1], ra[0], ra[1]};
+        Hh = u32x4{P[2][0], P[2][1], rb[0], rb[1]};
+    } else {
+        L = u32x4{ra[0], ra[1], P[1][0], P[1][1]};
+        Hh = u32x4{rb[0], rb[1], P[3][0], P[3][1]};
+    }
+    lo = __builtin_bit_cast(bf16x8, L);
+    hi = __builtin_bit_cast(bf16x8, Hh);
+}
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Asynchronous copy of rows [r0, r0 + 128) of a [256][256] bf16 weight into
+// the padded LDS image (global_load_lds_dwordx4: each active lane moves 16 B
+// straight into LDS; one row = 32 lanes).  Wave w moves rows r0 + 32w .. +31.
+__device__ __forceinline__ void stage_half_async(const __bf16 *W, __bf16 *wl, int r0, int wv, int lane) {
+    if (lane < 32) {
+#pragma unroll 8
+        for (int r = 0; r < 32; ++r) {
+            const int row = r0 + 32 * wv + r;
+            __builtin_amdgcn_global_load_lds((const void *)(W + row * kNH + lane * 8),
+                                             (lds_void *)(wl + row * kRowPad), 16, 0, 0);
+        }
+    }
+}
+
+// Output tiles nt0 .. nt0+3 of one hidden layer from the staged weight rows,
+// software-pipelined: the matrix-core products of tile nt run while the
+// vector units apply bias + SiLU and relayout tile nt - 1.
+template <int RBW>
+__device__ __forceinline__ void tile_mfma(const __bf16 *wl, int nt, int c, int h, const bf16x8 (&H)[RBW][16],
+                                          f32x16 (&acc)[RBW]) {
+#pragma unroll
+    for (int rb = 0; rb < RBW; ++rb)
+#pragma unroll
+        for (int j = 0; j < 16; ++j) acc[rb][j] = 0.0f;
+    const __bf16 *wrow = wl + (32 * nt + c) * kRowPad + 8 * h;
+#pragma unroll
+    for (int kb = 0; kb < 16; ++kb) {
+        const bf16x8 a = *(const bf16x8 *)(wrow + 16 * kb);
+#pragma unroll
+        for (int rb = 0; rb < RBW; ++rb)
+            acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, H[rb][kb], acc[rb], 0, 0, 0);
+    }
+}
+
+template <int RBW>
+__device__ __forceinline__ void hidden_half(const __bf16 *wl, const float *bias, int nt0, int c, int h,
+                                            const bf16x8 (&H)[RBW][16], bf16x8 (&Hn)[RBW][16]) {
+    f32x16 acc[2][RBW];
+    tile_mfma<RBW>(wl, nt0, c, h, H, acc[0]);
+#pragma unroll
+    for (int i = 1; i <= 4; ++i) {
+        if (i < 4) tile_mfma<RBW>(wl, nt0 + i, c, h, H, acc[i & 1]);
+        const int nt = nt0 + i - 1;
+#pragma unroll
+        for (int rb = 0; rb < RBW; ++rb)
+            tile_to_b(acc[(i - 1) & 1][rb], bias + 32 * nt, h, Hn[rb][2 * nt], Hn[rb][2 * nt + 1]);
+    }
+}
+
+__device__ __forceinline__ void wait_all_and_sync() {
+    __builtin_amdgcn_s_waitcnt(0);  // the LDS-direct loads of this wave have landed
+    __syncthreads();
+}
+
+// LDS image: rows 0..127 of the current weight ("A") and rows 128..255 ("B"),
+// 528-byte rows.  While one half is read by the matrix cores the other is
+// being refilled for the next step, so every weight load after the first
+// overlaps compute.  Layer 1's weights (16 KB) sit in B before layer 2's
+// second half arrives; layer 5's (4 KB) in A after layer 4's first half.
+template <int RBW>
+__global__ __launch_bounds__(256) void k_mlp_infer(const InferSet *__restrict__ sets, int nsets,
+                                                   FusedWeights fw, TargetNorm tn) {
+    extern __shared__ __attribute__((aligned(16))) __bf16 wl[];   // [256][kRowPad]
+    __shared__ float bs[3][kNH];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, h = lane >> 5, c = lane & 31;
+    // the set of this workgroup: last set whose first tile <= blockIdx.x
+    int lo = 0, hi = nsets - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sets[mid].tile0 <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
+    }
+    const InferSet &is = sets[lo];
+    const FeatSet fs = is.fs;
+    const int row0 = 0, n = fs.n4 + fs.SW * fs.SH;
+    float4 *__restrict__ S = is.S;
+    float *__restrict__ E = is.E;
+    const int rw = ((int)blockIdx.x - is.tile0) * (128 * RBW) + wv * (32 * RBW);    // first row of this wave
+    __bf16 *wB = wl + 128 * kRowPad;
+
+    // layer-1 weights (hi then lo, [256][16]) into B, hidden biases; layer 2's first half -> A
+    for (int i = threadIdx.x; i < 1024; i += 256)
+        *(uint4 *)(wB + i * 8) = *(const uint4 *)((i < 512 ? fw.a1h : fw.a1l) + (i & 511) * 8);
+    for (int i = threadIdx.x; i < 3 * kNH; i += 256) (&bs[0][0])[i] = fw.b234[i];
+    __syncthreads();
+    stage_half_async(fw.wt, wl, 0, wv, lane);
+
+    bf16x8 H[RBW][16];
+    // (1) features and layer 1
+#pragma unroll
+    for (int rb = 0; rb < RBW; ++rb) {
+        const int r = min(rw + 32 * rb + c, n - 1) + row0;
+        const unsigned code = r < fs.n4 ? (unsigned)r : (0x80000000u | (unsigned)(r - fs.n4));
+        float x[kNIN];
+        texel_features(fs, code, x);
+        float xe[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) xe[k] = k < kNIN ? x[k] : (k == kNIN ? 1.0f : 0.0f);
+        bf16x8 xh, xl;
+        const unsigned hm = 0u - (unsigned)h;  // all ones on lanes 32..63
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            // lane half h takes input columns 8h .. 8h+7 (bit select, no indexed array)
+            const float v = __uint_as_float((__float_as_uint(xe[i]) & ~hm) | (__float_as_uint(xe[8 + i]) & hm));
+            const __bf16 vh = (__bf16)v;
+            xh[i] = vh;
+            xl[i] = (__bf16)(v - (float)vh);
+        }
+#pragma unroll
+        for (int nt = 0; nt < 8; ++nt) {
+            const bf16x8 ah = *(const bf16x8 *)(wB + (32 * nt + c) * 16 + 8 * h);
+            const bf16x8 al = *(const bf16x8 *)(wB + 4096 + (32 * nt + c) * 16 + 8 * h);
+            f32x16 acc;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, xh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, xh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, xl, acc, 0, 0, 0);
+            tile_to_b(acc, nullptr, h, H[rb][2 * nt], H[rb][2 * nt + 1]);
+        }
+    }
+    // (2) hidden layers 2..4
+#pragma unroll 1
+    for (int L = 0; L < 3; ++L) {
+        const __bf16 *W = fw.wt + (size_t)L * kNH * kNH;
+        wait_all_and_sync();                          // A holds rows 0..127; B is free
+        stage_half_async(W, wl, 128, wv, lane);       // B <- rows 128..255
+        bf16x8 Hn[RBW][16];
+        hidden_half<RBW>(wl, bs[L], 0, c, h, H, Hn);
+        wait_all_and_sync();                          // B holds rows 128..255; A is free
+        if (L < 2) {
+            stage_half_async(W + (size_t)kNH * kNH, wl, 0, wv, lane);   // A <- next layer's rows 0..127
+        } else if (threadIdx.x < 256) {               // A <- layer 5 (hi then lo, [4][256])
+            *(uint4 *)(wl + threadIdx.x * 8) = *(const uint4 *)((threadIdx.x < 128 ? fw.a5h : fw.a5l) +
+                                                               (threadIdx.x & 127) * 8);
+        }
+        hidden_half<RBW>(wl, bs[L], 4, c, h, H, Hn);
+#pragma unroll
+        for (int rb = 0; rb < RBW; ++rb)
+#pragma unroll
+            for (int kb = 0; kb < 16; ++kb) H[rb][kb] = Hn[rb][kb];
+    }
+    // (3) layer 5 and the combination with the exact part
+    wait_all_and_sync();
+#pragma unroll
+    for (int rb = 0; rb < RBW; ++rb) {
+        f32x16 acc;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
+#pragma unroll
+        for (int kb = 0; kb < 16; ++kb) {
+            bf16x8 ah, al;
+            if (c < kNOUT) {
+                ah = *(const bf16x8 *)(wl + c * kNH + 16 * kb + 8 * h);
+                al = *(const bf16x8 *)(wl + 1024 + c * kNH + 16 * kb + 8 * h);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) ah[i] = al[i] = (__bf16)0.0f;
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, H[rb][kb], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, H[rb][kb], acc, 0, 0, 0);
+        }
+        // lanes h = 0 hold outputs 0..3 of row c in acc[0..3]
+        const int rr = rw + 32 * rb + c;
+        if (h == 0 && rr < n) {
+            const int r = row0 + rr;
+            float v[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                v[k] = fmaxf(__expf((acc[k] + fw.b5[k]) * tn.sd[k] + tn.mu[k]) - tn.eps, 0.f);
+            if (r < fs.n4) {
+                float4 sv = S[r];
+                sv.x += v[0]; sv.y += v[1]; sv.z += v[2];
+                S[r] = sv;
+            } else {
+                const int e = r - fs.n4;
+                E[3 * e] = v[0]; E[3 * e + 1] = v[1]; E[3 * e + 2] = v[2];
+            }
+        }
+    }
+}
+
+// Weights of the fused kernel from the FP32 parameters: layer 1 (+ bias in
+// column 11) and layer 5 split into bf16 hi/lo, the hidden biases packed.
+__global__ __launch_bounds__(256) void k_prep_fused(const float *__restrict__ W1, const float *__restrict__ b1,
+                                                    const float *__restrict__ W5, const float *__restrict__ b2,
+                                                    const float *__restrict__ b3, const float *__restrict__ b4,
+                                                    __bf16 *a1h, __bf16 *a1l, __bf16 *a5h, __bf16 *a5l,
+                                                    float *b234) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < kNH * 16) {
+        const int nn = i / 16, k = i % 16;
+        const float v = k < kNIN ? W1[k * kNH + nn] : (k == kNIN ? b1[nn] : 0.0f);
+        const __bf16 vh = (__bf16)v;
+        a1h[i] = vh;
+        a1l[i] = (__bf16)(v - (float)vh);
+    } else if (i < kNH * 16 + kNOUT * kNH) {
+        const int j = i - kNH * 16, o = j / kNH, k = j % kNH;
+        const float v = W5[k * kNOUT + o];
+        const __bf16 vh = (__bf16)v;
+        a5h[j] = vh;
+        a5l[j] = (__bf16)(v - (float)vh);
+    } else if (i < kNH * 16 + kNOUT * kNH + 3 * kNH) {
+        const int j = i - kNH * 16 - kNOUT * kNH;
+        b234[j] = j < kNH ? b2[j] : (j < 2 * kNH ? b3[j - kNH] : b4[j - 2 * kNH]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 hipError_t gemm(bool ta, bool tb, int epi, const __bf16 *A, const __bf16 *B, int M, int N, int K,
@@ -560,5 +834,34 @@ hipError_t absmax(const float *a, size_t n, int stride, int ncomp, unsigned *out
     return hipGetLastError();
 }
 
+}  // namespace sg
+}  // namespace atmo
+
+namespace atmo {
+namespace sg {
+constexpr int kFusedRBW = 2;
+size_t fused_lds() { return (size_t)kNH * kRowPad * sizeof(__bf16); }
+int fused_rows_per_block() { return 128 * kFusedRBW; }
+hipError_t prep_fused(const float *W1, const float *b1, const float *W5, const float *b2, const float *b3,
+                      const float *b4, __bf16 *a1h, __bf16 *a1l, __bf16 *a5h, __bf16 *a5l, float *b234,
+                      hipStream_t st) {
+    const int n = kNH * 16 + kNOUT * kNH + 3 * kNH;
+    hipLaunchKernelGGL(k_prep_fused, dim3(blocks(n)), dim3(256), 0, st, W1, b1, W5, b2, b3, b4, a1h, a1l, a5h,
+                       a5l, b234);
+    return hipGetLastError();
+}
+hipError_t mlp_infer(const InferSet *sets, int nsets, int ntiles, const FusedWeights &fw, const TargetNorm &tn,
+                     hipStream_t st) {
+    if (nsets <= 0 || ntiles <= 0) return hipSuccess;
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_mlp_infer<kFusedRBW>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)fused_lds());
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_mlp_infer<kFusedRBW>), dim3(ntiles), dim3(256), fused_lds(), st, sets, nsets, fw, tn);
+    return hipGetLastError();
+}
 }  // namespace sg
 }  // namespace atmo

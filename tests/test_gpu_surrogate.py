@@ -134,3 +134,38 @@ def test_judge_accepts_and_keeps_exact_parts(sur):
         assert np.all(np.isfinite(S)) and np.all(S >= 0)
         exact += np.array_equal(S, rS)
     assert exact == 2                                             # the judged sets
+
+
+def test_fused_inference_matches_the_network(sur):
+    """atmo_surrogate_precompute's fused kernel (features, five layers and the
+    combination in one launch per set) against the layer-by-layer network
+    (atmo_surrogate_predict on atmo_surrogate_features rows) plus the exact
+    single scattering, on every texel of two sets."""
+    sets = small_sets(2, seed=SEED_BATCH + 3)
+    mu0, sd0, eps0 = sur.get_norm()
+    sur.set_norm(np.array([-3.0, -2.5, -2.0], dtype=np.float32), np.array([1.0, 0.8, 1.2], dtype=np.float32), 1e-6)
+    try:
+        o = options()
+        outs, rep = sur.precompute(sets, o, judge_fraction=0.0)
+        torch.cuda.synchronize()
+        assert rep["judged"] == 0 and rep["accepted"] == 1 and rep["fallback_sets"] == 0
+        single = _plain(sets, options(orders=1))
+        mu, sd, eps = sur.get_norm()
+        for p, (T, E, S), (rT, rE, rS) in zip(sets, outs, single):
+            n4 = p.RES_R * p.RES_MU * p.RES_MU_S * p.RES_NU
+            nE = p.SKY_W * p.SKY_H
+            codes = np.concatenate([np.arange(n4, dtype=np.uint32),
+                                    np.uint32(0x80000000) | np.arange(nE, dtype=np.uint32)])
+            Y = sur.predict(SG.features(p, codes)).double().cpu().numpy()
+            v = np.maximum(np.exp(Y[:, :3] * sd.astype(np.float64) + mu) - eps, 0.0)
+            S = S.double().cpu().numpy().reshape(-1, 4)
+            vS = S[:, :3] - rS.reshape(-1, 4)[:, :3]
+            eS = np.abs(vS - v[:n4]) / np.maximum(v[:n4], 1e-3 * v[:n4].max())
+            eE = np.abs(E.double().cpu().numpy().reshape(-1, 3) - v[n4:]) / np.maximum(v[n4:], 1e-3 * v[n4:].max())
+            print("fused vs layer-by-layer: S max %.3e mean %.3e, E max %.3e" % (eS.max(), eS.mean(), eE.max()),
+                  flush=True)
+            assert eS.max() < 2e-2 and eS.mean() < 1e-3 and eE.max() < 2e-2
+            assert np.array_equal(S[:, 3], rS.reshape(-1, 4)[:, 3])
+            assert np.array_equal(T.double().cpu().numpy(), rT)
+    finally:
+        sur.set_norm(mu0, sd0, eps0)
