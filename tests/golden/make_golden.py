@@ -12,6 +12,12 @@ come from the pinned FP64 oracle (oracle/, checked in tests/test_oracle_pins.py)
   earth_ref.npz       the reference configuration (Earth, T 256x64, E 64x16,
                       inscatter 32x128x32x8, 500/50/32/16 samples, orders 4):
                       T and E in full, S at 4096 seeded texels
+  precision_hires.npz the precision-study configuration (Earth, T 512x128,
+                      E 128x32, inscatter 64x256x64x16, 32x64 directions,
+                      orders 8; FP64 throughout): T and E in full, S at 65536
+                      texels (a seeded subset of a seeded 2 M-texel sample).
+                      About 2.5 h of the oracle on 7 threads: run it alone,
+                      `python tests/golden/make_golden.py precision_hires`.
 """
 import os
 import sys
@@ -62,6 +68,34 @@ def earth_ref():
     flat = S.reshape(-1, 4)
     np.savez_compressed(os.path.join(HERE, "earth_ref.npz"), params=pvec(p), T=T.astype(np.float32),
                         E=E, S_idx=idx.astype(np.int32), S_val=flat[idx])
+
+
+def hires_params():
+    p = earth(TRANSMITTANCE_W=512, TRANSMITTANCE_H=128, SKY_W=128, SKY_H=32,
+              RES_R=64, RES_MU=256, RES_MU_S=64, RES_NU=16)
+    return p, options(orders=8, spherical_samples=32)
+
+
+def hires_sample(n4):
+    """S texels of the precision fixture: 65536 of a seeded 2 M-texel sample."""
+    idx = np.sort(np.random.default_rng(0x5EED0002).choice(n4, size=2_000_000, replace=False))
+    sub = np.sort(np.random.default_rng(0x5EED0006).choice(idx.size, size=65536, replace=False))
+    return idx[sub]
+
+
+def write_hires(p, T, E, S_idx, S_val, S_max):
+    np.savez_compressed(os.path.join(HERE, "precision_hires.npz"), params=pvec(p), T=T, E=E,
+                        S_idx=S_idx.astype(np.int64), S_val=S_val, S_max=S_max)
+
+
+def precision_hires():
+    p, o = hires_params()
+    t = time.time()
+    T, E, S = O.precompute(p, o)
+    print("precision-study oracle: %.1f s on %d threads" % (time.time() - t, O.get_threads()))
+    flat = S.reshape(-1, 4)
+    idx = hires_sample(flat.shape[0])
+    write_hires(p, T, E, idx, flat[idx], np.abs(S).max(axis=(0, 1, 2)))
 
 
 if __name__ == "__main__":

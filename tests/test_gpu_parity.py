@@ -329,3 +329,21 @@ def test_inscatterN_forms_agree_on_the_whole_chain(eng):
     assert torch.equal(a[0], b[0])  # transmittance does not involve dSn
     for x, y in zip(a, b):
         _assert("forms", y.double().cpu().numpy(), x.double().cpu().numpy(), 1e-5)
+
+
+def test_precision_study_configuration(eng):
+    """The precision-study workload (Earth, T 512x128, E 128x32, inscatter
+    64x256x64x16, 32x64 directions, orders 1-8) against the FP64 oracle:
+    T and E in full, S at the fixture's 65536 seeded texels; floor 1e-4 of each
+    channel's table maximum."""
+    g = np.load(os.path.join(GOLDEN, "precision_hires.npz"))
+    p = _params(g["params"])
+    o = options(orders=8, spherical_samples=32)
+    T, E, S = _chain(eng, p, o)
+    _assert("T", T, g["T"], TOL)
+    _assert("E", E, g["E"], TOL)
+    flat = S.reshape(-1, 4)[g["S_idx"]]
+    den = np.maximum(np.abs(g["S_val"]), 1e-4 * g["S_max"][None, :])
+    err = np.abs(flat - g["S_val"]) / den
+    print("S samples max %s mean %.3e" % (np.array2string(err.max(axis=0), precision=3), err.mean()), flush=True)
+    assert err.max() <= TOL
