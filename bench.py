@@ -21,6 +21,7 @@ for _p in (REPO, os.path.join(REPO, "proland-4.0_amd")):
         sys.path.insert(0, _p)
 
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (packed) peak, AMD public spec
+FP32_UNPACKED_PEAK_TFLOPS = 78.6   # the same without packed FP32 (1 FMA per lane per clock)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -164,11 +165,32 @@ def main():
         roof["executed_fp32_tflops"] = round(executed / (avg_ms * 1e-3) / 1e12, 2)
         roof["executed_frac"] = round(executed / (avg_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)
         roof["executed_source"] = COUNTER_PROFILE
+        # the 157.3 peak counts packed FP32, which this build does not emit
+        roof["executed_frac_of_unpacked_fp32_peak"] = round(
+            executed / (avg_ms * 1e-3) / 1e12 / FP32_UNPACKED_PEAK_TFLOPS, 4)
+    if traffic:
+        roof["hbm_gbs"] = round(traffic / (avg_ms * 1e-3) / 1e9, 1)
+        roof["hbm_frac"] = round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     whole_tflops = F.total(p, o) / sec / 1e12
+
+    # second kernel: the dSn pass, whose bound is its HBM traffic
+    tu = eng.tuning()
+    sn_name = "k_inscatterN<%s>" % ("true" if tu.get("inscatterN_staged_samples", 0) else "false")
+    sn_ms = [ms for name, order, ms in times if name == "inscatterN"]
+    sn_avg = sum(sn_ms) / len(sn_ms)
+    sn_traffic = profiled(TRAFFIC_PROFILE, "traffic", sn_name)
+    second = {"kernel": sn_name + " (dSn pass, orders 2-%d)" % o.orders, "bound": "hbm",
+              "avg_launch_ms": round(sn_avg, 4), "launches": len(sn_ms), "traffic": sn_traffic,
+              "unit": "GB/s", "peak": HBM_PEAK_GBS}
+    if sn_traffic:
+        second["achieved"] = round(sn_traffic / (sn_avg * 1e-3) / 1e9, 1)
+        second["frac"] = round(sn_traffic / (sn_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        second["traffic_source"] = TRAFFIC_PROFILE
 
     extra = {"pass_ms": {k: round(v, 3) for k, v in pass_ms.items()},
              "algorithmic_tflops_whole_precompute": round(whole_tflops, 2),
-             "autotuner": eng.tuning()}
+             "roofline_dsn": second,
+             "autotuner": tu}
     if not args.no_extra:
         # the reference configuration (Earth, 32x128x32x8, 4 orders) and its
         # accuracy at the committed FP64-oracle golden texels
