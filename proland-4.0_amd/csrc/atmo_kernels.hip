@@ -210,6 +210,7 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
                                                     const DirJ *__restrict__ dirT,
                                                     const float2 *__restrict__ dirW) {
     constexpr int NT = FIRST ? 2 : 1;
+    constexpr int PW = FIRST ? 3 : 2;      // float4 per staged pair entry
     extern __shared__ float4 lds[];
     const uint2 tl = tiles[blockIdx.x];
     const SetDesc &s = sets[tl.x];
@@ -218,8 +219,8 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
     const int NU = s.RNU;
     const int NP = NU + 1;                 // pairs per theta
     const int NV = NU + 2;                 // slice values per theta
-    float4 *pairs = lds;                                   // [NS][NP][NT][2]
-    float4 *vtmp = pairs + (size_t)NS * NP * NT * 2;       // [NS][NV][NT]
+    float4 *pairs = lds;                                   // [NS][NP][PW]
+    float4 *vtmp = pairs + (size_t)NS * NP * PW;           // [NS][NV][NT]
     float4 *epair = vtmp + (size_t)NS * NV * NT;           // [SW][2]
     float4 *gdat = epair + (size_t)s.SW * 2;               // [NS][2]
 
@@ -292,14 +293,23 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
     }
     __syncthreads();
     // (4) pairs {V[kk-1], V[kk] - V[kk-1]}
-    for (int e = threadIdx.x; e < NS * NP * NT; e += blockDim.x) {
-        const int n = e % NT, ek = e / NT;
-        const int it = ek / NP, kk = ek % NP;
-        float4 a = vtmp[((size_t)it * NV + kk) * NT + n];
-        float4 b = vtmp[((size_t)it * NV + kk + 1) * NT + n];
-        // packed as (V.rgb, D.r), (D.g, D.b): one 16-byte and one 8-byte LDS read
-        pairs[((size_t)(it * NP + kk) * NT + n) * 2] = make_float4(a.x, a.y, a.z, b.x - a.x);
-        pairs[((size_t)(it * NP + kk) * NT + n) * 2 + 1] = make_float4(b.y - a.y, b.z - a.z, 0.f, 0.f);
+    for (int e = threadIdx.x; e < NS * NP; e += blockDim.x) {
+        const int it = e / NP, kk = e % NP;
+        const float4 a = vtmp[((size_t)it * NV + kk) * NT];
+        const float4 b = vtmp[((size_t)it * NV + kk + 1) * NT];
+        float4 *q = pairs + (size_t)e * PW;
+        if (!FIRST) {
+            // (V.rgb, D.r), (D.g, D.b): one 16-byte and one 8-byte LDS read
+            q[0] = make_float4(a.x, a.y, a.z, b.x - a.x);
+            q[1] = make_float4(b.y - a.y, b.z - a.z, 0.f, 0.f);
+        } else {
+            // Rayleigh and Mie pairs in 48 bytes: three 16-byte LDS reads
+            const float4 am = vtmp[((size_t)it * NV + kk) * NT + 1];
+            const float4 bm = vtmp[((size_t)it * NV + kk + 1) * NT + 1];
+            q[0] = make_float4(a.x, a.y, a.z, b.x - a.x);
+            q[1] = make_float4(b.y - a.y, b.z - a.z, am.x, am.y);
+            q[2] = make_float4(am.z, bm.x - am.x, bm.y - am.y, bm.z - am.z);
+        }
     }
     __syncthreads();
 
@@ -345,27 +355,30 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
         // 4D lookup of one direction from the staged pairs: the clamp of the
         // nu slice is implicit, |s.w| <= 1 keeps pos in [1, RNU] (+ rounding)
         // and kk = trunc(pos) in [0, RNU] = the staged pair range.
-        auto lookup = [&](const float4 *pt, float pos, float out[3]) {
+        // (order 2: the Rayleigh and Mie values weighted by the phase functions of
+        // nu1 = s.w, as q1 (R + z1 M) with q1 = 1 + nu1^2, z1 = (1+g^2-2g nu1)^-3/2;
+        // the caller applies q1, whose per-set constants are folded into the staging)
+        auto lookup = [&](const float4 *pt, float pos, float out[3], float &q1) {
             const unsigned kk = __float2uint_rz(pos);
             const float f = __builtin_amdgcn_fractf(pos);
-            const float4 *pp = pt + kk * (NT * 2);
+            const float4 *pp = pt + kk * PW;
             const float4 a = pp[0];
-            const float2 d = *(const float2 *)(pp + 1);
-            out[0] = fmaf(f, a.w, a.x);
-            out[1] = fmaf(f, d.x, a.y);
-            out[2] = fmaf(f, d.y, a.z);
-            if (FIRST) {  // order 2: Rayleigh and Mie weighted by the phase of nu1 = s.w
-                const float4 am = pp[2];
-                const float2 dm = *(const float2 *)(pp + 3);
-                const float m0 = fmaf(f, am.w, am.x), m1 = fmaf(f, dm.x, am.y),
-                            m2 = fmaf(f, dm.y, am.z);
+            if (!FIRST) {
+                const float2 d = *(const float2 *)(pp + 1);
+                out[0] = fmaf(f, a.w, a.x);
+                out[1] = fmaf(f, d.x, a.y);
+                out[2] = fmaf(f, d.y, a.z);
+                q1 = 1.0f;
+            } else {
+                const float4 b = pp[1], c2 = pp[2];
                 const float nu1 = fmaf(pos, iposA, nposBi);
-                const float q1 = fmaf(nu1, nu1, 1.0f);
+                q1 = fmaf(nu1, nu1, 1.0f);
                 const float y1 = __builtin_amdgcn_rsqf(fmaf(m2g, nu1, g2p1));
-                const float t1 = q1 * (y1 * y1 * y1);
-                out[0] = fmaf(out[0], q1, m0 * t1);
-                out[1] = fmaf(out[1], q1, m1 * t1);
-                out[2] = fmaf(out[2], q1, m2 * t1);
+                const float z1 = y1 * y1 * y1;
+                const float m0 = fmaf(f, c2.y, b.z), m1 = fmaf(f, c2.z, b.w), m2 = fmaf(f, c2.w, c2.x);
+                out[0] = fmaf(m0, z1, fmaf(f, a.w, a.x));
+                out[1] = fmaf(m1, z1, fmaf(f, b.x, a.y));
+                out[2] = fmaf(m2, z1, fmaf(f, b.y, a.z));
             }
         };
         // irradiance at the ground point seen in direction w, row 0 of dE
@@ -387,7 +400,7 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
             const float vzwz = vz * ct;
             const float cx0 = fmaf(m2g, vzwz, g2p1);
             const float dw = g1.w, cp = g1.z;
-            const float4 *pt = pairs + (size_t)it * NP * NT * 2;
+            const float4 *pt = pairs + (size_t)it * NP * PW;
             const float2 *wrow = dirW + (size_t)it * 2 * NS;
             float2 wn = wrow[0];
             // directions phi and 2 pi - phi together: same w.x, opposite w.y,
@@ -408,11 +421,16 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
                     const float base = fmaf(sxp[g], w.x, cp);
                     pa[g] = fmaf(syp[g], w.y, base);
                     pb[g] = fmaf(-syp[g], w.y, base);
-                    float la[3], lb[3];
-                    lookup(pt, pa[g], la);
-                    lookup(pt, pb[g], lb);
+                    float la[3], lb[3], qa, qb;
+                    lookup(pt, pa[g], la, qa);
+                    lookup(pt, pb[g], lb, qb);
+                    if (FIRST) {
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) lv[g][c] = la[c] + lb[c];
+                        for (int c = 0; c < 3; ++c) lv[g][c] = fmaf(qa, la[c], qb * lb[c]);
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) lv[g][c] = la[c] + lb[c];
+                    }
                 }
                 if (ground) {
 #pragma unroll
@@ -878,8 +896,8 @@ __global__ __launch_bounds__(256) void k_check(const SetDesc *__restrict__ sets,
 // Host launchers
 // ---------------------------------------------------------------------------
 size_t lds_inscatterS(int NS, int RNU, int SW, bool first) {
-    const int NT = first ? 2 : 1;
-    return sizeof(float4) * ((size_t)NS * (RNU + 1) * NT * 2 + (size_t)NS * (RNU + 2) * NT +
+    const int NT = first ? 2 : 1, PW = first ? 3 : 2;
+    return sizeof(float4) * ((size_t)NS * (RNU + 1) * PW + (size_t)NS * (RNU + 2) * NT +
                              (size_t)SW * 2 + (size_t)NS * 2);
 }
 size_t lds_irradianceN(int NI, int RNU, bool first) {
