@@ -48,6 +48,7 @@ struct SetDesc {
     int n4;                         // RR * RMU * W4
     // tables (device pointers)
     float4 *T4, *dE4, *E4, *dSR4, *dSM4, *dJ4, *S4;
+    int dJb;                        // bytes per dJ texel: 16 (float4) or 12 (packed rgb)
     float *outT, *outE;
     const LayerGeom *layers;
 };

@@ -138,6 +138,7 @@ void fill_desc(SetDesc &s, const atmo_params &p) {
     s.HR = p.HR; s.HM = p.HM; s.mieG = p.mieG; s.alpha = p.AVERAGE_GROUND_REFLECTANCE;
     s.H = std::sqrt(s.Rt * s.Rt - s.Rg * s.Rg);
     s.Rg2 = s.Rg * s.Rg; s.Rt2 = s.Rt * s.Rt; s.RL2 = s.RL * s.RL;
+    s.dJb = 16;
     s.bR[0] = p.betaR.x; s.bR[1] = p.betaR.y; s.bR[2] = p.betaR.z;
     s.bMS[0] = p.betaMSca.x; s.bMS[1] = p.betaMSca.y; s.bMS[2] = p.betaMSca.z;
     s.bME[0] = p.betaMEx.x; s.bME[1] = p.betaMEx.y; s.bME[2] = p.betaMEx.z;
@@ -173,7 +174,7 @@ SetSizes sizes(const atmo_params &p) {
 size_t work_bytes(const atmo_params &p, const atmo_options &o) {
     SetSizes z = sizes(p);
     size_t b = align256(z.nT * 16) + 2 * align256(z.nE * 16) + 2 * align256(z.n4 * 16);
-    if (o.orders >= 2) b += align256(z.n4 * 16);  // dJ
+    if (o.orders >= 2) b += align256(z.n4 * 12);  // dJ, packed rgb
     return b;
 }
 
@@ -642,6 +643,7 @@ atmo_status build_plan(atmo_engine *e, const atmo_params *sets, int nsets, const
         s.dSR4 = (float4 *)b; b += align256(z.n4 * 16);
         s.dSM4 = (float4 *)b; b += align256(z.n4 * 16);
         s.dJ4 = o.orders >= 2 ? (float4 *)b : nullptr;
+        s.dJb = 12;  // the ray gathers of dSn read 12 bytes per texel
         s.layers = (const LayerGeom *)pl->dlayers.p + loff[i];
     }
     HIPCHK(pl->dsets.ensure(nsets * sizeof(SetDesc)));

@@ -456,7 +456,12 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
             const int k = kb + g;
             if (k < NU) {
                 const size_t o = ((size_t)l * s.RMU + y) * s.W4 + (size_t)k * s.RMUS + m;
-                s.dJ4[o] = make_float4(acc[g][0], acc[g][1], acc[g][2], 0.f);
+                if (s.dJb == 12) {
+                    float *q = (float *)s.dJ4 + 3 * o;
+                    q[0] = acc[g][0]; q[1] = acc[g][1]; q[2] = acc[g][2];
+                } else {
+                    s.dJ4[o] = make_float4(acc[g][0], acc[g][1], acc[g][2], 0.f);
+                }
             }
         }
     }
@@ -671,6 +676,7 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
     gchar *J = (gchar *)s.dJ4;
     const int W4m1 = s.W4 - 1;
     const unsigned W4m2 = (unsigned)(s.W4 - 2);
+    const unsigned jb = (unsigned)s.dJb;   // bytes per dJ texel
     for (int c0 = 0; c0 <= nsamp; c0 += kNChunk) {
         const int nc = min(kNChunk, nsamp + 1 - c0);
         __syncthreads();
@@ -718,8 +724,8 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
             const float wt = (i == 0 || i == nsamp) ? 0.5f : 1.0f;
             SnEntry en;
             en.w = make_float4((1.f - az) * (1.f - ay), (1.f - az) * ay, az * (1.f - ay), az * ay);
-            en.base = make_int4((z0 * HW + y0 * s.W4) * 16, (z0 * HW + y1 * s.W4) * 16,
-                                (z1 * HW + y0 * s.W4) * 16, (z1 * HW + y1 * s.W4) * 16);
+            en.base = make_int4((z0 * HW + y0 * s.W4) * s.dJb, (z0 * HW + y1 * s.W4) * s.dJb,
+                                (z1 * HW + y0 * s.W4) * s.dJb, (z1 * HW + y1 * s.W4) * s.dJb);
             en.tw = make_float4(Tseg.x * wt, Tseg.y * wt, Tseg.z * wt, (float)ti);
             en.ir = make_float4((float)(1.0 / ri), 0.f, 0.f, 0.f);
             ent[row][js] = en;
@@ -747,9 +753,9 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
                                             (unsigned)en.base.w};
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        const unsigned off = br[q] + i0 * 16u;
+                        const unsigned off = br[q] + i0 * jb;
                         const float3 c0v = ld3((const gfloat4 *)(J + off));
-                        const float3 c1v = ld3((const gfloat4 *)(J + off + 16u));
+                        const float3 c1v = ld3((const gfloat4 *)(J + off + jb));
                         const float wq = wr[q], wa = wr[q] * ax;
                         v.x = fmaf(wq, c0v.x, fmaf(wa, c1v.x - c0v.x, v.x));
                         v.y = fmaf(wq, c0v.y, fmaf(wa, c1v.y - c0v.y, v.y));
@@ -762,9 +768,9 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
                         float3 v2 = make_float3(0.f, 0.f, 0.f);
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
-                            const unsigned off = br[q] + j0 * 16u;
+                            const unsigned off = br[q] + j0 * jb;
                             const float3 c0v = ld3((const gfloat4 *)(J + off));
-                            const float3 c1v = ld3((const gfloat4 *)(J + off + 16u));
+                            const float3 c1v = ld3((const gfloat4 *)(J + off + jb));
                             const float wq = wr[q], wa = wr[q] * bx;
                             v2.x = fmaf(wq, c0v.x, fmaf(wa, c1v.x - c0v.x, v2.x));
                             v2.y = fmaf(wq, c0v.y, fmaf(wa, c1v.y - c0v.y, v2.y));
@@ -789,7 +795,7 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
                         const int e = threadIdx.x + j * (int)blockDim.x;
                         if (e >= per) break;
                         const SnEntry &en = ent[rowi[j]][s0 + k];
-                        const unsigned xo = (unsigned)(e - rowi[j] * s.W4) * 16u;
+                        const unsigned xo = (unsigned)(e - rowi[j] * s.W4) * jb;
                         const float3 t0 = ld3((const gfloat4 *)(J + (unsigned)en.base.x + xo));
                         const float3 t1 = ld3((const gfloat4 *)(J + (unsigned)en.base.y + xo));
                         const float3 t2 = ld3((const gfloat4 *)(J + (unsigned)en.base.z + xo));
