@@ -352,7 +352,13 @@ void build_tiles(const std::vector<atmo_params> &ps, const atmo_options &o,
             switch (pass) {
             case P_T: nt = (z.nT + 15) / 16; break;
             case P_E1: nt = (z.nE + 255) / 256; break;
-            case P_S1: nt = (z.n4 + 255) / 256; break;
+            case P_S1: {
+                // workgroups of up to 4 mu rows (k_inscatter1)
+                const int W4 = p.RES_MU_S * p.RES_NU;
+                const int rpw = std::max(1, std::min(4, 256 / W4));
+                nt = (long long)p.RES_R * ((p.RES_MU + rpw - 1) / rpw);
+                break;
+            }
             case P_SN: {
                 // workgroups of up to snCap mu rows (k_inscatterN)
                 const int W4 = p.RES_MU_S * p.RES_NU;

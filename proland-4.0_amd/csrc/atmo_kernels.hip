@@ -86,66 +86,160 @@ __global__ __launch_bounds__(256) void k_irradiance1(const SetDesc *__restrict__
     }
 }
 
+constexpr int kNChunk = 64;   // samples staged per chunk
+constexpr int kNRows = 4;     // max mu rows per workgroup
+
+// rows per workgroup: as many whole rows as fit 256 threads, at most rpw_cap (<= 4)
+__device__ __forceinline__ int sn_rows_per_wg(int W4, int rpw_cap) {
+    return max(1, min(min(rpw_cap, kNRows), 256 / W4));
+}
+
 // ---------------------------------------------------------------------------
 // Pass dS1 — inscatter1.glsl:104-150 fused with copyInscatter1.glsl:104-109.
-// One thread per 4D texel; tile = 256 texels of one set.
+// A texel's view ray depends only on its r layer and mu row, so everything of
+// an integration sample that does not involve the sun — its distance and
+// radius, the view-ray transmittance T(r, mu, t), the scale-height factors and
+// the r-row of the T(r_i, mu_s_i) lookup — is shared by the RNU x RMU_S texels
+// of the row.  A workgroup covers up to 4 rows (as k_inscatterN), stages those
+// terms (FP64 geometry) in LDS, and each thread then needs only the sun
+// visibility test, mu_s_i and the mu-axis half of the T lookup per sample.
 // ---------------------------------------------------------------------------
+struct S1Entry {
+    double ti;       // sample distance
+    double q;        // r_i^2 - Rg^2
+    float T[3];      // T(r, mu, t_i), trapezoid weight folded in
+    float iri;       // 1 / r_i (FP32)
+    float eR, eM;    // exp(-h_i/HR), exp(-h_i/HM)
+    float b;         // T-table row weight of r_i
+    int j0, j1;      // T-table row offsets (texels) of r_i
+    int pad;
+};
+
+constexpr int kS1Chunk = 64;
+
 __global__ __launch_bounds__(256) void k_inscatter1(const SetDesc *__restrict__ sets,
                                                     const uint2 *__restrict__ tiles,
                                                     int nsamp) {
+    __shared__ S1Entry ent[kNRows][kS1Chunk];
+    __shared__ float rowdx[kNRows];
     const uint2 tl = tiles[blockIdx.x];
     const SetDesc &s = sets[tl.x];
-    const int t = (int)tl.y * 256 + threadIdx.x;
-    if (t >= s.n4) return;
-    const int x = t % s.W4;
-    const int y = (t / s.W4) % s.RMU;
-    const int l = t / (s.W4 * s.RMU);
+    const int RPW = sn_rows_per_wg(s.W4, kNRows);
+    const int ngy = (s.RMU + RPW - 1) / RPW;
+    const int l = (int)tl.y / ngy;
+    const int yb = ((int)tl.y % ngy) * RPW;
+    const int nrows = min(RPW, s.RMU - yb);
     const LayerGeom L = s.layers[l];
-    double mu, muS, nu;
-    mu_mus_nu(s, L, x, y, mu, muS, nu);
     const double r = L.r;
-    const double rmu = r * mu;
     const double rho2 = r * r - s.Rg2;
-    const double dx = limit_d(s, r, mu) / nsamp;
-    const bool up = mu > 0.0;
-    const float3 Tfix = T_lookup(s, (float)(r - s.Rg), (float)(up ? mu : -mu));
-    const double muSr = muS * r;
-    const float iHR = (float)(1.0 / s.HR), iHM = (float)(1.0 / s.HM);
-    float3 aR = make_float3(0.f, 0.f, 0.f), aM = make_float3(0.f, 0.f, 0.f);
-    for (int i = 0; i <= nsamp; ++i) {
-        const double ti = i * dx;
-        const double q = rho2 + ti * (ti + 2.0 * rmu);  // ri^2 - Rg^2
-        const double num = fma(nu, ti, muSr);          // ri * muSi
-        // muSi >= -sqrt(1 - Rg^2 / max(Rg, ri)^2)  <=>  num >= 0 or num^2 <= ri^2 - Rg^2
-        if (!(num >= 0.0 || num * num <= q)) continue;
-        const float ri = sqrtf((float)(q + s.Rg2));
-        const float h1 = (float)q / (ri + s.f_Rg);      // ri - Rg (may be < 0 at a ground hit)
-        const float mu1 = (float)(rmu + ti) / ri;
-        float3 Tseg;
-        if (up) {
-            float3 b = T_lookup(s, h1, mu1);
-            Tseg = make_float3(fminf(Tfix.x / b.x, 1.f), fminf(Tfix.y / b.y, 1.f),
-                               fminf(Tfix.z / b.z, 1.f));
-        } else {
-            float3 a = T_lookup(s, h1, -mu1);
-            Tseg = make_float3(fminf(a.x / Tfix.x, 1.f), fminf(a.y / Tfix.y, 1.f),
-                               fminf(a.z / Tfix.z, 1.f));
+    const int per = nrows * s.W4;
+    const int nmine = (per + (int)blockDim.x - 1) / (int)blockDim.x;
+    constexpr int MAXT = 4;   // W4 <= 1024 (validated on the host)
+    float aR[MAXT][3], aM[MAXT][3];
+    double nud[MAXT], muSr[MAXT];
+    int rowi[MAXT];
+#pragma unroll
+    for (int j = 0; j < MAXT; ++j) {
+        aR[j][0] = aR[j][1] = aR[j][2] = aM[j][0] = aM[j][1] = aM[j][2] = 0.f;
+        nud[j] = muSr[j] = 0.0;
+        rowi[j] = 0;
+        const int e = threadIdx.x + j * (int)blockDim.x;
+        if (j < nmine && e < per) {
+            const int row = e / s.W4, x = e % s.W4;
+            double mu, muS, nu;
+            mu_mus_nu(s, L, x, yb + row, mu, muS, nu);
+            nud[j] = nu;
+            muSr[j] = muS * r;
+            rowi[j] = row;
         }
-        const float hc = fmaxf(h1, 0.0f);              // ri = max(Rg, ri)
-        const float muSi = (float)num / ri;
-        const float3 Ts = T_lookup(s, hc, muSi);
-        const float w = (i == 0 || i == nsamp) ? 0.5f : 1.0f;
-        const float eR = w * __expf(-hc * iHR), eM = w * __expf(-hc * iHM);
-        const float3 tt = Tseg * Ts;
-        aR = aR + tt * eR;
-        aM = aM + tt * eM;
     }
-    const float fdx = (float)dx;
-    float4 oR = make_float4(aR.x * fdx * s.bR[0], aR.y * fdx * s.bR[1], aR.z * fdx * s.bR[2], 0.f);
-    float4 oM = make_float4(aM.x * fdx * s.bMS[0], aM.y * fdx * s.bMS[1], aM.z * fdx * s.bMS[2], 0.f);
-    s.dSR4[t] = oR;
-    s.dSM4[t] = oM;
-    if (s.S4) s.S4[t] = make_float4(oR.x, oR.y, oR.z, oM.x);
+    const float iHR = (float)(1.0 / s.HR), iHM = (float)(1.0 / s.HM);
+    const gfloat4 *T4 = gptr(s.T4);
+    for (int c0 = 0; c0 <= nsamp; c0 += kS1Chunk) {
+        const int nc = min(kS1Chunk, nsamp + 1 - c0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < nrows * kS1Chunk; e += blockDim.x) {
+            const int row = e / kS1Chunk, js = e % kS1Chunk;
+            if (js >= nc) continue;
+            const int i = c0 + js;
+            double mu, muS_u, nu_u;
+            mu_mus_nu(s, L, 0, yb + row, mu, muS_u, nu_u);
+            const double rmu = r * mu;
+            const double dx = limit_d(s, r, mu) / nsamp;
+            if (i == 0) rowdx[row] = (float)dx;
+            const bool up = mu > 0.0;
+            const float3 Tfix = T_lookup(s, (float)(r - s.Rg), (float)(up ? mu : -mu));
+            S1Entry en;
+            en.ti = i * dx;
+            en.q = rho2 + en.ti * (en.ti + 2.0 * rmu);                 // ri^2 - Rg^2
+            const float ri = sqrtf((float)(en.q + s.Rg2));
+            const float h1 = (float)en.q / (ri + s.f_Rg);              // ri - Rg (< 0 at a ground hit)
+            const float mu1 = (float)(rmu + en.ti) / ri;
+            float3 Tseg;
+            if (up) {
+                const float3 b = T_lookup(s, h1, mu1);
+                Tseg = make_float3(fminf(Tfix.x / b.x, 1.f), fminf(Tfix.y / b.y, 1.f), fminf(Tfix.z / b.z, 1.f));
+            } else {
+                const float3 a = T_lookup(s, h1, -mu1);
+                Tseg = make_float3(fminf(a.x / Tfix.x, 1.f), fminf(a.y / Tfix.y, 1.f), fminf(a.z / Tfix.z, 1.f));
+            }
+            const float w = (i == 0 || i == nsamp) ? 0.5f : 1.0f;
+            en.T[0] = Tseg.x * w; en.T[1] = Tseg.y * w; en.T[2] = Tseg.z * w;
+            en.iri = 1.0f / ri;
+            const float hc = fmaxf(h1, 0.0f);                          // ri = max(Rg, ri)
+            en.eR = __expf(-hc * iHR);
+            en.eM = __expf(-hc * iHM);
+            // r axis of T(max(Rg, ri), muSi): common.glsl:114-124
+            int j0, j1;
+            gl_axis(sqrtf(hc * s.f_inv_dRt) * s.TH - 0.5f, s.TH, j0, j1, en.b);
+            en.j0 = j0 * s.TW;
+            en.j1 = j1 * s.TW;
+            en.pad = 0;
+            ent[row][js] = en;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < MAXT; ++j) {
+            if (j >= nmine) break;
+            const int e = threadIdx.x + j * (int)blockDim.x;
+            if (e >= per) break;
+            const S1Entry *er = ent[rowi[j]];
+            float r0 = aR[j][0], r1 = aR[j][1], r2 = aR[j][2];
+            float m0 = aM[j][0], m1 = aM[j][1], m2 = aM[j][2];
+            for (int js = 0; js < nc; ++js) {
+                const S1Entry &en = er[js];
+                const double num = fma(nud[j], en.ti, muSr[j]);        // ri * muSi
+                // muSi >= -sqrt(1 - Rg^2 / max(Rg, ri)^2)  <=>  num >= 0 or num^2 <= ri^2 - Rg^2
+                if (!(num >= 0.0 || num * num <= en.q)) continue;
+                const float muSi = (float)num * en.iri;
+                int i0, i1;
+                float a;
+                gl_axis(atanf((muSi + 0.15f) * s.f_tMu) * (1.0f / 1.5f) * s.TW - 0.5f, s.TW, i0, i1, a);
+                const float3 t00 = ld3(T4 + en.j0 + i0), t01 = ld3(T4 + en.j0 + i1);
+                const float3 t10 = ld3(T4 + en.j1 + i0), t11 = ld3(T4 + en.j1 + i1);
+                const float3 Ts = lerp3(lerp3(t00, t01, a), lerp3(t10, t11, a), en.b);
+                const float tx = en.T[0] * Ts.x, ty = en.T[1] * Ts.y, tz = en.T[2] * Ts.z;
+                r0 = fmaf(tx, en.eR, r0); r1 = fmaf(ty, en.eR, r1); r2 = fmaf(tz, en.eR, r2);
+                m0 = fmaf(tx, en.eM, m0); m1 = fmaf(ty, en.eM, m1); m2 = fmaf(tz, en.eM, m2);
+            }
+            aR[j][0] = r0; aR[j][1] = r1; aR[j][2] = r2;
+            aM[j][0] = m0; aM[j][1] = m1; aM[j][2] = m2;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < MAXT; ++j) {
+        if (j >= nmine) break;
+        const int e = threadIdx.x + j * (int)blockDim.x;
+        if (e >= per) break;
+        const int row = e / s.W4, x = e % s.W4;
+        const size_t t = ((size_t)l * s.RMU + yb + row) * s.W4 + x;
+        const float fdx = rowdx[row];
+        const float4 oR = make_float4(aR[j][0] * fdx * s.bR[0], aR[j][1] * fdx * s.bR[1], aR[j][2] * fdx * s.bR[2], 0.f);
+        const float4 oM = make_float4(aM[j][0] * fdx * s.bMS[0], aM[j][1] * fdx * s.bMS[1], aM[j][2] * fdx * s.bMS[2], 0.f);
+        s.dSR4[t] = oR;
+        s.dSM4[t] = oM;
+        if (s.S4) s.S4[t] = make_float4(oR.x, oR.y, oR.z, oM.x);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -600,14 +694,6 @@ __global__ __launch_bounds__(256) void k_irradianceN(const SetDesc *__restrict__
 // those per-sample terms in LDS (FP64 geometry, in chunks of 64 samples) and
 // each thread then integrates its texels with only the mu_s-dependent work.
 // ---------------------------------------------------------------------------
-constexpr int kNChunk = 64;   // samples staged per chunk
-constexpr int kNRows = 4;     // max mu rows per workgroup
-
-// rows per workgroup: as many whole rows as fit 256 threads, at most rpw_cap (<= 4)
-__device__ __forceinline__ int sn_rows_per_wg(int W4, int rpw_cap) {
-    return max(1, min(min(rpw_cap, kNRows), 256 / W4));
-}
-
 struct SnEntry {
     float4 w;     // bilinear (r, mu) weights of the 4 table rows
     int4 base;    // offsets of the 4 rows (z0y0, z0y1, z1y0, z1y1)
