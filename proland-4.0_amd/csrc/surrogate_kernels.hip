@@ -533,12 +533,14 @@ typedef __attribute__((address_space(3))) void lds_void;
 
 // Asynchronous copy of rows [r0, r0 + 128) of a [256][256] bf16 weight into
 // the padded LDS image (global_load_lds_dwordx4: each active lane moves 16 B
-// straight into LDS; one row = 32 lanes).  Wave w moves rows r0 + 32w .. +31.
+// straight into LDS; one row = 32 lanes).  Each of the NW waves moves 128/NW rows.
+template <int NW>
 __device__ __forceinline__ void stage_half_async(const __bf16 *W, __bf16 *wl, int r0, int wv, int lane) {
+    constexpr int RPWV = 128 / NW;
     if (lane < 32) {
 #pragma unroll 8
-        for (int r = 0; r < 32; ++r) {
-            const int row = r0 + 32 * wv + r;
+        for (int r = 0; r < RPWV; ++r) {
+            const int row = r0 + RPWV * wv + r;
             __builtin_amdgcn_global_load_lds((const void *)(W + row * kNH + lane * 8),
                                              (lds_void *)(wl + row * kRowPad), 16, 0, 0);
         }
@@ -590,9 +592,9 @@ __device__ __forceinline__ void wait_all_and_sync() {
 // being refilled for the next step, so every weight load after the first
 // overlaps compute.  Layer 1's weights (16 KB) sit in B before layer 2's
 // second half arrives; layer 5's (4 KB) in A after layer 4's first half.
-template <int RBW>
-__global__ __launch_bounds__(256) void k_mlp_infer(const InferSet *__restrict__ sets, int nsets,
-                                                   FusedWeights fw, TargetNorm tn) {
+template <int RBW, int NW>
+__global__ __launch_bounds__(64 * NW) void k_mlp_infer(const InferSet *__restrict__ sets, int nsets,
+                                                       FusedWeights fw, TargetNorm tn) {
     extern __shared__ __attribute__((aligned(16))) __bf16 wl[];   // [256][kRowPad]
     __shared__ float bs[3][kNH];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, h = lane >> 5, c = lane & 31;
@@ -607,15 +609,15 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const InferSet *__restrict__ 
     const int row0 = 0, n = fs.n4 + fs.SW * fs.SH;
     float4 *__restrict__ S = is.S;
     float *__restrict__ E = is.E;
-    const int rw = ((int)blockIdx.x - is.tile0) * (128 * RBW) + wv * (32 * RBW);    // first row of this wave
+    const int rw = ((int)blockIdx.x - is.tile0) * (32 * RBW * NW) + wv * (32 * RBW);    // first row of this wave
     __bf16 *wB = wl + 128 * kRowPad;
 
     // layer-1 weights (hi then lo, [256][16]) into B, hidden biases; layer 2's first half -> A
-    for (int i = threadIdx.x; i < 1024; i += 256)
+    for (int i = threadIdx.x; i < 1024; i += 64 * NW)
         *(uint4 *)(wB + i * 8) = *(const uint4 *)((i < 512 ? fw.a1h : fw.a1l) + (i & 511) * 8);
-    for (int i = threadIdx.x; i < 3 * kNH; i += 256) (&bs[0][0])[i] = fw.b234[i];
+    for (int i = threadIdx.x; i < 3 * kNH; i += 64 * NW) (&bs[0][0])[i] = fw.b234[i];
     __syncthreads();
-    stage_half_async(fw.wt, wl, 0, wv, lane);
+    stage_half_async<NW>(fw.wt, wl, 0, wv, lane);
 
     bf16x8 H[RBW][16];
     // (1) features and layer 1
@@ -656,12 +658,12 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const InferSet *__restrict__ 
     for (int L = 0; L < 3; ++L) {
         const __bf16 *W = fw.wt + (size_t)L * kNH * kNH;
         wait_all_and_sync();                          // A holds rows 0..127; B is free
-        stage_half_async(W, wl, 128, wv, lane);       // B <- rows 128..255
+        stage_half_async<NW>(W, wl, 128, wv, lane);   // B <- rows 128..255
         bf16x8 Hn[RBW][16];
         hidden_half<RBW>(wl, bs[L], 0, c, h, H, Hn);
         wait_all_and_sync();                          // B holds rows 128..255; A is free
         if (L < 2) {
-            stage_half_async(W + (size_t)kNH * kNH, wl, 0, wv, lane);   // A <- next layer's rows 0..127
+            stage_half_async<NW>(W + (size_t)kNH * kNH, wl, 0, wv, lane);   // A <- next layer's rows 0..127
         } else if (threadIdx.x < 256) {               // A <- layer 5 (hi then lo, [4][256])
             *(uint4 *)(wl + threadIdx.x * 8) = *(const uint4 *)((threadIdx.x < 128 ? fw.a5h : fw.a5l) +
                                                                (threadIdx.x & 127) * 8);
@@ -839,9 +841,9 @@ hipError_t absmax(const float *a, size_t n, int stride, int ncomp, unsigned *out
 
 namespace atmo {
 namespace sg {
-constexpr int kFusedRBW = 2;
+constexpr int kFusedRBW = 1, kFusedNW = 8;
 size_t fused_lds() { return (size_t)kNH * kRowPad * sizeof(__bf16); }
-int fused_rows_per_block() { return 128 * kFusedRBW; }
+int fused_rows_per_block() { return 32 * kFusedRBW * kFusedNW; }
 hipError_t prep_fused(const float *W1, const float *b1, const float *W5, const float *b2, const float *b3,
                       const float *b4, __bf16 *a1h, __bf16 *a1l, __bf16 *a5h, __bf16 *a5l, float *b234,
                       hipStream_t st) {
@@ -855,12 +857,13 @@ hipError_t mlp_infer(const InferSet *sets, int nsets, int ntiles, const FusedWei
     if (nsets <= 0 || ntiles <= 0) return hipSuccess;
     static bool attr = false;
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_mlp_infer<kFusedRBW>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_mlp_infer<kFusedRBW, kFusedNW>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)fused_lds());
         if (e != hipSuccess) return e;
         attr = true;
     }
-    hipLaunchKernelGGL((k_mlp_infer<kFusedRBW>), dim3(ntiles), dim3(256), fused_lds(), st, sets, nsets, fw, tn);
+    hipLaunchKernelGGL((k_mlp_infer<kFusedRBW, kFusedNW>), dim3(ntiles), dim3(64 * kFusedNW), fused_lds(), st, sets,
+                       nsets, fw, tn);
     return hipGetLastError();
 }
 }  // namespace sg
