@@ -294,6 +294,10 @@ __device__ __forceinline__ void stage_slice(const SetDesc &s, const float4 *cons
 // posA s'.w + posB, with s' = posA s folded in per texel and the mu_s term
 // folded in per theta.
 // ---------------------------------------------------------------------------
+// Entries of padding in front of the staged ground-irradiance row: texel
+// coordinates down to -(0.8/1.2) SW - 0.5 (mu_s = -1) plus rounding.
+__host__ __device__ __forceinline__ int e_pad(int SW) { return (2 * SW) / 3 + 3; }
+
 struct DirJ {      // per-theta constants of the NS x 2NS grid
     float ct, st, dw, pad;
 };
@@ -315,8 +319,9 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
     const int NV = NU + 2;                 // slice values per theta
     float4 *pairs = lds;                                   // [NS][NP][PW]
     float4 *vtmp = pairs + (size_t)NS * NP * PW;           // [NS][NV][NT]
-    float4 *epair = vtmp + (size_t)NS * NV * NT;           // [SW][2]
-    float4 *gdat = epair + (size_t)s.SW * 2;               // [NS][2]
+    const int EPAD = e_pad(s.SW);
+    float4 *epair = vtmp + (size_t)NS * NV * NT;           // [EPAD + SW + 1][2]
+    float4 *gdat = epair + (size_t)(EPAD + s.SW + 1) * 2;  // [NS][2]
 
     const LayerGeom L = s.layers[l];
     const double r = fmin(fmax(L.r, s.Rg), s.Rt);          // inscatterS.glsl:112
@@ -342,10 +347,15 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
 #pragma unroll
         for (int n = 0; n < NT; ++n) vtmp[(size_t)e * NT + n] = v[n];
     }
-    // (2) row 0 of dE (irradiance at r = Rg reads only row 0 under CLAMP_TO_EDGE)
-    for (int k = threadIdx.x; k < s.SW; k += blockDim.x) {
-        float4 a = s.dE4[k];
-        float4 b = s.dE4[min(k + 1, s.SW - 1)];
+    // (2) row 0 of dE (irradiance at r = Rg reads only row 0 under CLAMP_TO_EDGE),
+    // with EPAD copies of texel 0 in front and one of texel SW-1 behind (zero
+    // differences): every texel coordinate the ground term can produce,
+    // (mu_s + 0.2)/1.2 SW - 0.5 for |mu_s| <= 1, then lands on an entry that
+    // reproduces the clamp, and the lookup needs no clamp instruction
+    for (int k = threadIdx.x; k < EPAD + s.SW + 1; k += blockDim.x) {
+        const int k0 = min(max(k - EPAD, 0), s.SW - 1), k1 = min(max(k - EPAD + 1, 0), s.SW - 1);
+        const float4 a = s.dE4[k0];
+        const float4 b = s.dE4[k1];
         epair[2 * k] = make_float4(a.x, a.y, a.z, b.x - a.x);
         epair[2 * k + 1] = make_float4(b.y - a.y, b.z - a.z, 0.f, 0.f);
     }
@@ -377,7 +387,7 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
             g0 = make_float4(gr * tr.x, gr * tr.y, gr * tr.z, 1.0f);
             // E texel coordinate ((r muS + dg nu1)/Rg + 0.2)/1.2 SW - 0.5 as c1 pos + c0
             const double sc = s.SW / 1.2;
-            const double e1 = dg / s.Rg * sc, e0 = (r * muS / s.Rg + 0.2) * sc - 0.5;
+            const double e1 = dg / s.Rg * sc, e0 = (r * muS / s.Rg + 0.2) * sc - 0.5 + EPAD;
             c1 = e1 / posAd;
             c0 = e0 - e1 * posBd / posAd;
         }
@@ -414,7 +424,6 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
     const float B0 = s.bMS[0] * eM * s.kM, B1 = s.bMS[1] * eM * s.kM, B2 = s.bMS[2] * eM * s.kM;
     const float iposA = (float)(1.0 / posAd), nposBi = (float)(-posBd / posAd);
     const float g2p1 = s.g2p1, m2g = s.m2g;
-    const float SWm1 = (float)(s.SW - 1);
     const int ngk = (NU + G - 1) / G;
     const int ngroups = s.RMU * ngk;
 
@@ -477,7 +486,7 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
         };
         // irradiance at the ground point seen in direction w, row 0 of dE
         auto ground_e = [&](float pos, float cg1, float cg0, float out[3]) {
-            const float xg = __builtin_amdgcn_fmed3f(fmaf(pos, cg1, cg0), 0.0f, SWm1);
+            const float xg = fmaf(pos, cg1, cg0);   // >= 0: EPAD is folded into cg0
             const unsigned kg = __float2uint_rz(xg);
             const float fg = __builtin_amdgcn_fractf(xg);
             const float4 ea = epair[2 * kg];
@@ -990,7 +999,7 @@ __global__ __launch_bounds__(256) void k_check(const SetDesc *__restrict__ sets,
 size_t lds_inscatterS(int NS, int RNU, int SW, bool first) {
     const int NT = first ? 2 : 1, PW = first ? 3 : 2;
     return sizeof(float4) * ((size_t)NS * (RNU + 1) * PW + (size_t)NS * (RNU + 2) * NT +
-                             (size_t)SW * 2 + (size_t)NS * 2);
+                             (size_t)(e_pad(SW) + SW + 1) * 2 + (size_t)NS * 2);
 }
 size_t lds_irradianceN(int NI, int RNU, bool first) {
     const int NT = first ? 2 : 1;
