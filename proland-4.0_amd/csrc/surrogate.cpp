@@ -391,18 +391,16 @@ atmo_status atmo_surrogate_build_dataset(atmo_surrogate *s, const atmo_params *s
         HIPCHK(hipStreamSynchronize(s->st));
     }
     s->nrows = row;
-    // standardise: statistics of every 7th row
-    std::vector<float> h;
-    const size_t stride = 7, ns = (row + stride - 1) / stride;
-    h.resize(ns * kNOUT);
-    HIPCHK(hipMemcpy2D(h.data(), kNOUT * 4, s->Ta.p, stride * kNOUT * 4, kNOUT * 4, ns, hipMemcpyDeviceToHost));
+    // standardise: mean and deviation of every target row (GPU reduction in FP64)
+    Buf stats;
+    HIPCHK(stats.ensure(6 * sizeof(double)));
+    HIPCHK(hipMemsetAsync(stats.p, 0, 6 * sizeof(double), s->st));
+    HIPCHK(target_stats(s->Ta.as<float>(), row, (double *)stats.p, s->st));
+    double hs[6];
+    HIPCHK(hipMemcpyAsync(hs, stats.p, sizeof hs, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
     for (int c = 0; c < 3; ++c) {
-        double sm = 0, sq = 0;
-        for (size_t i = 0; i < ns; ++i) {
-            sm += h[i * kNOUT + c];
-            sq += (double)h[i * kNOUT + c] * h[i * kNOUT + c];
-        }
-        const double mu = sm / ns, var = std::max(sq / ns - mu * mu, 1e-6);
+        const double mu = hs[c] / (double)row, var = std::max(hs[3 + c] / (double)row - mu * mu, 1e-6);
         s->tn.mu[c] = (float)mu;
         s->tn.sd[c] = (float)std::sqrt(var);
     }

@@ -68,6 +68,8 @@ hipError_t make_rows(const FeatSet &fs, const float4 *S4, const float4 *S1, cons
                      hipStream_t st);
 hipError_t infer_rows(const FeatSet &fs, int row0, int n, float *X, hipStream_t st);
 hipError_t code_rows(const FeatSet &fs, const unsigned *codes, int n, float *X, hipStream_t st);
+// out[6] (device, zeroed by the caller): sums and sums of squares of the rgb targets
+hipError_t target_stats(const float *Tg, size_t n, double *out, hipStream_t st);
 hipError_t norm_targets(float *Tg, size_t n, const TargetNorm &tn, hipStream_t st);
 hipError_t apply(int row0, int n, int n4, const float *Y, const TargetNorm &tn, float4 *S, float *E,
                  hipStream_t st);

@@ -389,6 +389,32 @@ __global__ __launch_bounds__(256) void k_code_rows(FeatSet fs, const unsigned *_
     for (int i = kNIN; i < kXS; ++i) X[(size_t)t * kXS + i] = 0.f;
 }
 
+// Sum and sum of squares of the three rgb targets over n rows (FP64, for the
+// output standardisation): out[0..2] sums, out[3..5] sums of squares.
+__global__ __launch_bounds__(256) void k_target_stats(const float *__restrict__ Tg, size_t n,
+                                                      double *__restrict__ out) {
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (size_t)gridDim.x * 256) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double v = Tg[t * kNOUT + c];
+            a[c] += v;
+            a[3 + c] += v * v;
+        }
+    }
+    __shared__ double red[6][256];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = a[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) atomicAdd(&out[threadIdx.x], red[threadIdx.x][0]);
+}
+
 // In-place standardisation of the three rgb targets.
 __global__ __launch_bounds__(256) void k_norm_targets(float *__restrict__ Tg, size_t n, TargetNorm tn) {
     const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -810,6 +836,10 @@ hipError_t infer_rows(const FeatSet &fs, int row0, int n, float *X, hipStream_t 
 }
 hipError_t code_rows(const FeatSet &fs, const unsigned *codes, int n, float *X, hipStream_t st) {
     hipLaunchKernelGGL(k_code_rows, dim3(blocks(n)), dim3(256), 0, st, fs, codes, n, X);
+    return hipGetLastError();
+}
+hipError_t target_stats(const float *Tg, size_t n, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_target_stats, dim3(1024), dim3(256), 0, st, Tg, n, out);
     return hipGetLastError();
 }
 hipError_t norm_targets(float *Tg, size_t n, const TargetNorm &tn, hipStream_t st) {
