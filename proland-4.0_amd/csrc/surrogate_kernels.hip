@@ -7,6 +7,8 @@
 // layers, the loss, the reductions and Adam are FP32 vector code.
 #include "surrogate_kernels.h"
 
+#include <mutex>
+
 namespace atmo {
 namespace sg {
 
@@ -885,12 +887,21 @@ hipError_t prep_fused(const float *W1, const float *b1, const float *W5, const f
 hipError_t mlp_infer(const InferSet *sets, int nsets, int ntiles, const FusedWeights &fw, const TargetNorm &tn,
                      hipStream_t st) {
     if (nsets <= 0 || ntiles <= 0) return hipSuccess;
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_mlp_infer<kFusedRBW, kFusedNW>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)fused_lds());
-        if (e != hipSuccess) return e;
-        attr = true;
+    // the kernel takes more dynamic LDS than the default limit: raise it once per device
+    static std::mutex mu;
+    static unsigned long long done = 0;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!(done & (1ull << dev))) {
+            e = hipFuncSetAttribute((const void *)k_mlp_infer<kFusedRBW, kFusedNW>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)fused_lds());
+            if (e != hipSuccess) return e;
+            done |= 1ull << dev;
+        }
     }
     hipLaunchKernelGGL((k_mlp_infer<kFusedRBW, kFusedNW>), dim3(ntiles), dim3(64 * kFusedNW), fused_lds(), st, sets,
                        nsets, fw, tn);
