@@ -166,16 +166,17 @@ ATMO_API atmo_status atmo_engine_pass_times(atmo_engine *e, atmo_pass_time *out,
  * tables; the two k_inscatterN forms (direct gathers / LDS-staged blended
  * rows) compute the same trilinear filter in a different FP32 order.  Times in
  * ms: inscatterS_ms[i] for 2 / 4 / 8 texels per thread (one order-2 +
- * (orders-2) later launches), inscatterN_ms[6f + 2c + x] for form f (direct,
- * staged), at most 1 / 2 / 4 mu rows per workgroup (c), without / with
- * XCD-banded tile order (x); -1 = does not fit the device's LDS. */
+ * (orders-2) later launches), inscatterN_ms[9f + 3c + x] for form f (direct,
+ * staged), at most 1 / 2 / 4 mu rows per workgroup (c), tile order x (0 mu
+ * rows, 1 XCD-banded mu rows, 2 XCD-banded view lines sorted by impact
+ * parameter); -1 = does not fit the device's LDS. */
 typedef struct atmo_tuning {
     int tuned;
     int inscatterS_texels_per_thread;
     int inscatterN_rows_per_workgroup;
-    int inscatterN_xcd_order;
+    int inscatterN_xcd_order;      /* 0, 1 or 2 (see inscatterN_ms) */
     float inscatterS_ms[3];
-    float inscatterN_ms[12];
+    float inscatterN_ms[18];
     int inscatterN_staged_samples;  /* 0: direct gathers; > 0: staged, samples per LDS round */
 } atmo_tuning;
 ATMO_API atmo_status atmo_engine_tuning(atmo_engine *e, atmo_tuning *out);

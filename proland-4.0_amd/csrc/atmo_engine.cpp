@@ -132,6 +132,24 @@ LayerGeom layer_geom(const atmo_params &p, int l) {
     return g;
 }
 
+// mu of texel row y of a layer (getMuMuSNu, common.glsl:175-190), as the kernels compute it
+double row_mu(const atmo_params &p, const LayerGeom &g, int y) {
+    const double r = g.r, Rg = p.Rg, Rt = p.Rt, half = p.RES_MU * 0.5;
+    double mu;
+    if (y < half) {
+        double d = 1.0 - y / (half - 1.0);
+        d = std::min(std::max(g.dhdH[2], d * g.dhdH[3]), g.dhdH[3] * 0.999);
+        mu = (Rg * Rg - r * r - d * d) / (2.0 * r * d);
+        const double q = Rg / r;
+        mu = std::min(mu, -std::sqrt(1.0 - q * q) - 0.001);
+    } else {
+        double d = (y - half) / (half - 1.0);
+        d = std::min(std::max(g.dhdH[0], d * g.dhdH[1]), g.dhdH[1] * 0.999);
+        mu = (Rt * Rt - r * r - d * d) / (2.0 * r * d);
+    }
+    return std::min(std::max(mu, -1.0), 1.0);
+}
+
 void fill_desc(SetDesc &s, const atmo_params &p) {
     memset(&s, 0, sizeof s);
     s.Rg = p.Rg; s.Rt = p.Rt; s.RL = p.RL;
@@ -207,7 +225,7 @@ struct DevBuf {
 struct Tuning {
     int jG = 4, snCap = 4, snXcd = 1, snSb = 0;
     float msJ[3] = {0, 0, 0};     // G = 2, 4, 8 (order-2 + order->=3 launches)
-    float msN[12] = {};           // [direct, staged 2][caps 1, 2, 4][plain, XCD order]
+    float msN[18] = {};           // [direct, staged 2][caps 1, 2, 4][plain, XCD, XCD + line order]
 };
 
 // static LDS of k_inscatterN (sample staging), on top of its dynamic row blend
@@ -380,10 +398,32 @@ void build_tiles(const std::vector<atmo_params> &ps, const atmo_options &o,
                              [](const std::pair<double, uint2> &a, const std::pair<double, uint2> &b) {
                                  return a.first > b.first;
                              });
+        if (pass == P_SN && snXcd == 2) {
+            // "line order": a view ray is a straight line, fixed by its impact
+            // parameter r sqrt(1 - mu^2) and its half (up / down); rays of different
+            // layers on nearby lines gather the same dJ rows.  Sort the tiles by
+            // (set, half, impact parameter) before dealing them to the XCDs.
+            std::vector<std::pair<std::pair<double, double>, uint2>> k;
+            k.reserve(v.size());
+            for (auto &e : v) {
+                const atmo_params &p = ps[e.second.x];
+                const int W4 = p.RES_MU_S * p.RES_NU;
+                const int rpw = std::max(1, std::min(std::min(snCap, 4), 256 / W4));
+                const int ngy = (p.RES_MU + rpw - 1) / rpw;
+                const int l = (int)e.second.y / ngy, yb = ((int)e.second.y % ngy) * rpw;
+                const LayerGeom g = layer_geom(p, l);
+                const double mu = row_mu(p, g, yb);
+                const double half = yb < p.RES_MU / 2 ? 0.0 : 1.0;
+                k.push_back({{e.second.x * 2.0 + half, g.r * std::sqrt(std::max(0.0, 1.0 - mu * mu))}, e.second});
+            }
+            std::stable_sort(k.begin(), k.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+            for (size_t i = 0; i < v.size(); ++i) v[i].second = k[i].second;
+        }
         if (pass == P_SN && snXcd) {
             // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2);
-            // deal them so that XCD x runs a contiguous band of mu rows, whose rays
-            // gather from overlapping parts of the dJ table.
+            // deal them so that XCD x runs a contiguous band of tiles (mu rows, or
+            // lines in line order), whose rays gather from overlapping parts of the
+            // dJ table.
             const size_t nt = v.size(), chunk = (nt + kXcds - 1) / kXcds;
             std::vector<std::pair<double, uint2>> w;
             w.reserve(nt);
@@ -563,10 +603,10 @@ atmo_status autotune(Plan &pl, hipStream_t st, Tuning &tu) {
     float bestN = 1e30f;
     for (int si = 0; si < 2; ++si)
     for (int ci = 0; ci < 3; ++ci)
-        for (int xo = 0; xo < 2; ++xo) {
+        for (int xo = 0; xo < 3; ++xo) {
             const size_t lds = lds_inscatterN(sn_max_row_texels(pl.params, caps[ci]), sbs[si]);
             if (lds + kSnStaticLds > (size_t)maxLds) {  // does not fit this device's LDS
-                tu.msN[si * 6 + ci * 2 + xo] = -1.f;
+                tu.msN[si * 9 + ci * 3 + xo] = -1.f;
                 continue;
             }
             std::vector<uint2> tiles;
@@ -580,7 +620,7 @@ atmo_status autotune(Plan &pl, hipStream_t st, Tuning &tu) {
                                          o.inscatter_samples, caps[ci], sbs[si], lds, st);
             });
             if (rc != ATMO_OK) return rc;
-            tu.msN[si * 6 + ci * 2 + xo] = ms;
+            tu.msN[si * 9 + ci * 3 + xo] = ms;
             if (ms < bestN) {
                 bestN = ms;
                 tu.snCap = caps[ci];
@@ -764,7 +804,7 @@ atmo_status atmo_engine_tuning(atmo_engine *e, atmo_tuning *out) {
     out->inscatterN_rows_per_workgroup = e->last_tuning.snCap;
     out->inscatterN_xcd_order = e->last_tuning.snXcd;
     for (int i = 0; i < 3; ++i) out->inscatterS_ms[i] = e->last_tuning.msJ[i];
-    for (int i = 0; i < 12; ++i) out->inscatterN_ms[i] = e->last_tuning.msN[i];
+    for (int i = 0; i < 18; ++i) out->inscatterN_ms[i] = e->last_tuning.msN[i];
     out->inscatterN_staged_samples = e->last_tuning.snSb;
     return ATMO_OK;
 }
