@@ -78,9 +78,9 @@ typedef struct atmo_options {
     int irradiance_samples;     /* IRRADIANCE_INTEGRAL_SAMPLES (32) */
     int spherical_samples;      /* INSCATTER_SPHERICAL_INTEGRAL_SAMPLES (16) */
     int flags;                  /* ATMO_FLAG_* */
-    int texels_per_thread;      /* k_inscatterS variant 2, 4 or 8 (bit-identical results);
-                                   0 = 4 for the pass-level entry points, autotuned in
-                                   atmo_engine_precompute */
+    int texels_per_thread;      /* k_inscatterS variant 2, 3, 4 or 8 (bit-identical results);
+                                   0 = per set by workgroup occupancy for the pass-level
+                                   entry points, autotuned in atmo_engine_precompute */
     int inscatterN_form;        /* k_inscatterN form: 1 direct gathers, 2 LDS-staged rows;
                                    0 = direct for the pass-level entry points, autotuned
                                    in atmo_engine_precompute */
@@ -162,20 +162,21 @@ ATMO_API atmo_status atmo_engine_pass_times(atmo_engine *e, atmo_pass_time *out,
 /* Kernel variants the autotuner picked for the engine's current plan.  After
  * the first run of a batch shape not seen before in the process, the engine
  * times every variant of the two dominant passes and keeps the fastest.
- * k_inscatterS variants (2 / 4 / 8 texels per thread) give bit-identical
- * tables; the two k_inscatterN forms (direct gathers / LDS-staged blended
+ * k_inscatterS variants (2 / 3 / 4 / 8 texels per thread, or each set with
+ * the G whose workgroup-occupancy cost ceil(RES_MU * ceil(RES_NU / G) / 256) * G
+ * is lowest, reported as 0) give bit-identical tables; the two k_inscatterN forms (direct gathers / LDS-staged blended
  * rows) compute the same trilinear filter in a different FP32 order.  Times in
- * ms: inscatterS_ms[i] for 2 / 4 / 8 texels per thread (one order-2 +
+ * ms: inscatterS_ms[i] for 2 / 3 / 4 / 8 / per-set texels per thread (one order-2 +
  * (orders-2) later launches), inscatterN_ms[9f + 3c + x] for form f (direct,
  * staged), at most 1 / 2 / 4 mu rows per workgroup (c), tile order x (0 mu
  * rows, 1 XCD-banded mu rows, 2 XCD-banded view lines sorted by impact
  * parameter); -1 = does not fit the device's LDS. */
 typedef struct atmo_tuning {
     int tuned;
-    int inscatterS_texels_per_thread;
+    int inscatterS_texels_per_thread;  /* 2, 3, 4, 8; 0: per set */
     int inscatterN_rows_per_workgroup;
     int inscatterN_xcd_order;      /* 0, 1 or 2 (see inscatterN_ms) */
-    float inscatterS_ms[3];
+    float inscatterS_ms[5];
     float inscatterN_ms[18];
     int inscatterN_staged_samples;  /* 0: direct gathers; > 0: staged, samples per LDS round */
 } atmo_tuning;

@@ -102,9 +102,9 @@ atmo_status validate_opts(const atmo_options &o) {
         return fail(ATMO_ERR_INVALID_ARGUMENT, "irradiance_samples must be even, in [2, 256]");
     if (o.spherical_samples < 1 || o.spherical_samples > 128)
         return fail(ATMO_ERR_INVALID_ARGUMENT, "spherical_samples must be in [1, 128]");
-    if (o.texels_per_thread != 0 && o.texels_per_thread != 2 && o.texels_per_thread != 4 &&
-        o.texels_per_thread != 8)
-        return fail(ATMO_ERR_INVALID_ARGUMENT, "texels_per_thread must be 0, 2, 4 or 8");
+    if (o.texels_per_thread != 0 && o.texels_per_thread != 2 && o.texels_per_thread != 3 &&
+        o.texels_per_thread != 4 && o.texels_per_thread != 8)
+        return fail(ATMO_ERR_INVALID_ARGUMENT, "texels_per_thread must be 0, 2, 3, 4 or 8");
     if (o.inscatterN_form < 0 || o.inscatterN_form > 2)
         return fail(ATMO_ERR_INVALID_ARGUMENT, "inscatterN_form must be 0, 1 or 2");
     return ATMO_OK;
@@ -221,10 +221,33 @@ struct DevBuf {
     }
 };
 
+// k_inscatterS texels per thread for one set: a workgroup integrates the
+// RES_MU x RES_NU texels of one (r, mu_s) column, in RES_MU * ceil(RES_NU/G)
+// groups of G texels over 256 threads, so its time is about
+// ceil(groups / 256) * G; the smallest such G in {4, 3, 2} (4 on ties, whose
+// per-direction terms are shared most widely).
+int jg_for_set(const atmo_params &p) {
+    int best = 4;
+    long best_t = 1L << 40;
+    for (int g : {4, 3, 2}) {
+        const long groups = (long)p.RES_MU * ((p.RES_NU + g - 1) / g);
+        const long t = ((groups + 255) / 256) * g;
+        if (t < best_t) {
+            best_t = t;
+            best = g;
+        }
+    }
+    return best;
+}
+
+struct JSeg {   // a run of P_J tiles whose sets share one G
+    int G, off, cnt;
+};
+
 // Kernel variants chosen by the autotuner, with the times measured for each.
 struct Tuning {
-    int jG = 4, snCap = 4, snXcd = 1, snSb = 0;
-    float msJ[3] = {0, 0, 0};     // G = 2, 4, 8 (order-2 + order->=3 launches)
+    int jG = 4, snCap = 4, snXcd = 1, snSb = 0;   // jG = 0: per-set choice
+    float msJ[5] = {0, 0, 0, 0, 0};  // G = 2, 3, 4, 8, per set (order-2 + order->=3 launches)
     float msN[18] = {};           // [direct, staged 2][caps 1, 2, 4][plain, XCD, XCD + line order]
 };
 
@@ -254,7 +277,8 @@ struct Plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     // kernel variants (picked by the autotuner)
-    int jG = 4;        // k_inscatterS texels per thread
+    int jG = 4;        // k_inscatterS texels per thread (0: per set, see jg_for_set)
+    std::vector<JSeg> jseg;   // P_J tiles grouped by the per-set G
     int snCap = 4;     // k_inscatterN max mu rows per workgroup
     int snXcd = 1;     // k_inscatterN XCD-banded tile order
     int snSb = 0;      // k_inscatterN: 0 direct gathers, > 0 LDS-staged rows (samples per round)
@@ -356,7 +380,7 @@ atmo_status upload_dirs(DevBuf &buf, const atmo_options &o, size_t &offJT, size_
 // the most expensive tiles first so mixed-resolution batches drain evenly.
 void build_tiles(const std::vector<atmo_params> &ps, const atmo_options &o,
                  std::vector<uint2> &tiles, int off[P_COUNT], int cnt[P_COUNT], int snCap = 4,
-                 int snXcd = 1) {
+                 int snXcd = 1, std::vector<JSeg> *jseg = nullptr) {
     tiles.clear();
     const int n = (int)ps.size();
     for (int pass = 0; pass < P_COUNT; ++pass) {
@@ -384,7 +408,11 @@ void build_tiles(const std::vector<atmo_params> &ps, const atmo_options &o,
                 nt = (long long)p.RES_R * ((p.RES_MU + rpw - 1) / rpw);
                 break;
             }
-            case P_J: nt = (long long)p.RES_R * p.RES_MU_S; cost = (double)p.RES_MU * p.RES_NU; break;
+            case P_J:
+                // grouped by the per-set G (descending), most expensive first inside a group
+                nt = (long long)p.RES_R * p.RES_MU_S;
+                cost = jg_for_set(p) * 1e12 + (double)p.RES_MU * p.RES_NU;
+                break;
             case P_EN: nt = (long long)z.nE; cost = (double)p.RES_NU; break;
             case P_CHECK: nt = (std::max(std::max(z.n4, z.nT), z.nE) + 255) / 256; break;
             }
@@ -433,9 +461,34 @@ void build_tiles(const std::vector<atmo_params> &ps, const atmo_options &o,
             }
             v.swap(w);
         }
+        if (pass == P_J && jseg) {
+            jseg->clear();
+            for (size_t i = 0; i < v.size(); ++i) {
+                const int g = jg_for_set(ps[v[i].second.x]);
+                if (jseg->empty() || jseg->back().G != g) jseg->push_back({g, (int)i, 0});
+                jseg->back().cnt += 1;
+            }
+        }
         for (auto &e : v) tiles.push_back(e.second);
         cnt[pass] = (int)tiles.size() - off[pass];
     }
+}
+
+// The dJ pass of a plan: one launch with G texels per thread, or with G = 0
+// one launch per run of sets sharing their own G.
+hipError_t launch_dj(const Plan &pl, const SetDesc *S, const uint2 *T, const char *dirs, bool first, int G,
+                     hipStream_t st) {
+    const atmo_options &o = pl.opts;
+    const size_t lds = first ? pl.ldsJ1 : pl.ldsJ;
+    if (G != 0)
+        return launch_inscatterS(S, T + pl.off[P_J], pl.cnt[P_J], o.spherical_samples, dirs + pl.dirJT_off,
+                                 dirs + pl.dirJW_off, first, lds, G, st);
+    for (const JSeg &g : pl.jseg) {
+        hipError_t e = launch_inscatterS(S, T + pl.off[P_J] + g.off, g.cnt, o.spherical_samples,
+                                         dirs + pl.dirJT_off, dirs + pl.dirJW_off, first, lds, g.G, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // Event pairs around each pass launch (ATMO_FLAG_TIME_PASSES).
@@ -485,9 +538,7 @@ atmo_status launch_all(Plan &pl, hipStream_t st, PassTimer *tm = nullptr) {
     for (int order = 2; order <= o.orders; ++order) {
         const bool first = order == 2;
         HIPCHK(t.begin(3, order));
-        HIPCHK(launch_inscatterS(S, T + pl.off[P_J], pl.cnt[P_J], o.spherical_samples,
-                                 dirs + pl.dirJT_off, dirs + pl.dirJW_off, first,
-                                 first ? pl.ldsJ1 : pl.ldsJ, pl.jG, st));
+        HIPCHK(launch_dj(pl, S, T, dirs, first, pl.jG, st));
         HIPCHK(t.end());
         HIPCHK(t.begin(4, order));
         HIPCHK(launch_irradianceN(S, T + pl.off[P_EN], pl.cnt[P_EN], o.irradiance_samples,
@@ -569,19 +620,13 @@ atmo_status autotune(Plan &pl, hipStream_t st, Tuning &tu) {
     const uint2 *T = (const uint2 *)pl.dtiles.p;
     const char *dirs = (const char *)pl.ddirs.p;
     HIPCHK(hipStreamSynchronize(st));
-    const int Gs[3] = {2, 4, 8};
+    const int Gs[5] = {2, 3, 4, 8, 0};   // 0: each set with its own G (jg_for_set)
     float bestJ = 1e30f;
-    for (int gi = 0; gi < 3; ++gi) {
+    for (int gi = 0; gi < 5; ++gi) {
         float m1 = 0.f, m2 = 0.f;
-        atmo_status rc = time_launch(st, m1, [&]() {
-            return launch_inscatterS(S, T + pl.off[P_J], pl.cnt[P_J], o.spherical_samples, dirs + pl.dirJT_off,
-                                     dirs + pl.dirJW_off, true, pl.ldsJ1, Gs[gi], st);
-        });
+        atmo_status rc = time_launch(st, m1, [&]() { return launch_dj(pl, S, T, dirs, true, Gs[gi], st); });
         if (rc != ATMO_OK) return rc;
-        rc = time_launch(st, m2, [&]() {
-            return launch_inscatterS(S, T + pl.off[P_J], pl.cnt[P_J], o.spherical_samples, dirs + pl.dirJT_off,
-                                     dirs + pl.dirJW_off, false, pl.ldsJ, Gs[gi], st);
-        });
+        rc = time_launch(st, m2, [&]() { return launch_dj(pl, S, T, dirs, false, Gs[gi], st); });
         if (rc != ATMO_OK) return rc;
         // weight: one order-2 launch, (orders - 2) later ones
         tu.msJ[gi] = m1 + m2 * std::max(0, o.orders - 2);
@@ -642,7 +687,7 @@ atmo_status apply_tuning(Plan &pl, const Tuning &tu) {
     pl.tuned = true;
     if (retile) {
         std::vector<uint2> tiles;
-        build_tiles(pl.params, pl.opts, tiles, pl.off, pl.cnt, pl.snCap, pl.snXcd);
+        build_tiles(pl.params, pl.opts, tiles, pl.off, pl.cnt, pl.snCap, pl.snXcd, &pl.jseg);
         HIPCHK(pl.dtiles.ensure(tiles.size() * sizeof(uint2)));
         HIPCHK(hipMemcpy(pl.dtiles.p, tiles.data(), tiles.size() * sizeof(uint2), hipMemcpyHostToDevice));
     }
@@ -655,7 +700,7 @@ atmo_status build_plan(atmo_engine *e, const atmo_params *sets, int nsets, const
     std::unique_ptr<Plan> pl(new Plan);
     pl->params.assign(sets, sets + nsets);
     pl->opts = o;
-    pl->jG = o.texels_per_thread ? o.texels_per_thread : 4;
+    pl->jG = o.texels_per_thread;   // 0: each set with its own G until the tuner has run
     pl->snSb = o.inscatterN_form == 2 ? kSnStagedSamples : 0;
     // work arena
     size_t total = 0;
@@ -695,7 +740,7 @@ atmo_status build_plan(atmo_engine *e, const atmo_params *sets, int nsets, const
     HIPCHK(pl->dsets.ensure(nsets * sizeof(SetDesc)));
     // tiles
     std::vector<uint2> tiles;
-    build_tiles(pl->params, o, tiles, pl->off, pl->cnt);
+    build_tiles(pl->params, o, tiles, pl->off, pl->cnt, 4, 1, &pl->jseg);
     HIPCHK(pl->dtiles.ensure(tiles.size() * sizeof(uint2)));
     HIPCHK(hipMemcpy(pl->dtiles.p, tiles.data(), tiles.size() * sizeof(uint2), hipMemcpyHostToDevice));
     HIPCHK(pl->dflags.ensure(nsets * sizeof(unsigned)));
@@ -803,7 +848,7 @@ atmo_status atmo_engine_tuning(atmo_engine *e, atmo_tuning *out) {
     out->inscatterS_texels_per_thread = e->last_tuning.jG;
     out->inscatterN_rows_per_workgroup = e->last_tuning.snCap;
     out->inscatterN_xcd_order = e->last_tuning.snXcd;
-    for (int i = 0; i < 3; ++i) out->inscatterS_ms[i] = e->last_tuning.msJ[i];
+    for (int i = 0; i < 5; ++i) out->inscatterS_ms[i] = e->last_tuning.msJ[i];
     for (int i = 0; i < 18; ++i) out->inscatterN_ms[i] = e->last_tuning.msN[i];
     out->inscatterN_staged_samples = e->last_tuning.snSb;
     return ATMO_OK;
@@ -996,7 +1041,7 @@ static atmo_status pass_run(atmo_engine *e, const atmo_params *p, const atmo_opt
         size_t lds = lds_inscatterS(o.spherical_samples, p->RES_NU, p->SKY_W, first);
         if (lds > 160 * 1024) return fail(ATMO_ERR_UNSUPPORTED, "LDS staging too large");
         HIPCHK(launch_inscatterS(S, T + off[P_J], cnt[P_J], o.spherical_samples, d + oJT,
-                                 d + oJW, first, lds, o.texels_per_thread ? o.texels_per_thread : 4, hs));
+                                 d + oJW, first, lds, o.texels_per_thread ? o.texels_per_thread : jg_for_set(*p), hs));
         break;
     }
     case P_EN: {

@@ -24,7 +24,7 @@ hipError_t launch_transmittance(const SetDesc *sets, const uint2 *tiles, int nti
 hipError_t launch_irradiance1(const SetDesc *sets, const uint2 *tiles, int ntiles, hipStream_t st);
 hipError_t launch_inscatter1(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
                              hipStream_t st);
-// G: texels per thread (2, 4 or 8)
+// G: texels per thread (2, 3, 4 or 8)
 hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS,
                              const void *dirT, const void *dirW, bool first, size_t lds, int G,
                              hipStream_t st);

@@ -1034,6 +1034,8 @@ hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles
                        (const DirJ *)dirT, (const float2 *)dirW)
     if (G == 2) {
         if (first) ATMO_LJ(true, 2); else ATMO_LJ(false, 2);
+    } else if (G == 3) {
+        if (first) ATMO_LJ(true, 3); else ATMO_LJ(false, 3);
     } else if (G == 4) {
         if (first) ATMO_LJ(true, 4); else ATMO_LJ(false, 4);
     } else if (G == 8) {

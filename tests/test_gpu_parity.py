@@ -276,7 +276,7 @@ def test_autotuned_variants_agree_with_the_defaults(res):
     torch.cuda.synchronize()
     t = e2.tuning()
     print(t, flush=True)
-    assert t["tuned"] == 1 and t["inscatterS_texels_per_thread"] in (2, 4, 8)
+    assert t["tuned"] == 1 and t["inscatterS_texels_per_thread"] in (0, 2, 3, 4, 8)
     assert t["inscatterN_rows_per_workgroup"] in (1, 2, 4)
     assert t["inscatterN_staged_samples"] in (0, 2)
     for a, b in zip(ref[0], outs[0]):
@@ -290,12 +290,29 @@ def test_autotuned_variants_agree_with_the_defaults(res):
     e2.close()
 
 
+def test_per_set_inscatterS_variants_in_one_batch(eng):
+    """Untuned, a batch runs k_inscatterS with each set's own texels per thread
+    (3 for RES_MU 64 x RES_NU 12, 2 for 16 x 4, 4 for 128 x 8), one launch per
+    group of sets; every set's tables equal its single run with G pinned to 4."""
+    sets = []
+    for res in ((4, 64, 6, 12), (4, 16, 6, 4), (4, 128, 4, 8), (4, 64, 6, 12)):
+        q = small(res=res)
+        sets.append(q)
+    outs = eng.precompute(sets, options(flags=8))
+    torch.cuda.synchronize()
+    for q, o in zip(sets, outs):
+        one = eng.precompute(q, options(flags=8, texels_per_thread=4))[0]
+        torch.cuda.synchronize()
+        for a, b in zip(o, one):
+            assert torch.equal(a, b)
+
+
 @pytest.mark.parametrize("res", [(8, 32, 16, 8), (6, 16, 6, 5)])
 def test_every_inscatterS_variant_gives_identical_tables(eng, res):
-    """texels_per_thread 2, 4 and 8 pin the k_inscatterS variant; the tables
+    """texels_per_thread 2, 3, 4 and 8 pin the k_inscatterS variant; the tables
     must agree bit for bit (RES_NU = 5 leaves ragged texel groups)."""
     p = small(res=res)
-    outs = [eng.precompute(p, options(flags=8, texels_per_thread=g))[0] for g in (2, 4, 8)]
+    outs = [eng.precompute(p, options(flags=8, texels_per_thread=g))[0] for g in (2, 3, 4, 8)]
     torch.cuda.synchronize()
     for o in outs[1:]:
         for a, b in zip(outs[0], o):
