@@ -92,6 +92,12 @@ typedef struct atmo_options {
 #define ATMO_FLAG_TIME_PASSES    4  /* eager launches, a hipEvent pair around each pass
                                        (read with atmo_engine_pass_times) */
 #define ATMO_FLAG_NO_AUTOTUNE    8  /* keep the default kernel variants */
+#define ATMO_FLAG_HALF_STORAGE  16  /* store every table value rounded to the nearest FP16
+                                       (round to nearest even, gradual underflow), and every
+                                       ADD blend as FP16(dst + src), as the reference's
+                                       RGB16F / RGBA16F textures and blending do
+                                       (PreprocessAtmo.cpp:130-143, :336, :350); the
+                                       default keeps FP32 tables */
 
 /* Reference defaults (PreprocessAtmo.cpp:90-98; note betaMEx = 4.44e-3). */
 ATMO_API void atmo_params_default(atmo_params *p);

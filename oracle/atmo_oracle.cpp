@@ -681,26 +681,60 @@ void oracle_pass_copy_irradiance(const atmo_params *a, double k, const double *d
     }
 }
 
+/* Nearest IEEE binary16 value (round to nearest even, gradual underflow below
+ * 2^-14 on the quantum 2^-24, overflow to +-inf from 65520): what storing a
+ * value into an RGB16F / RGBA16F texture keeps (GL 3.3 core 2.1.2). */
+double oracle_q16(double x) {
+    const double ax = std::fabs(x);
+    if (std::isnan(x)) return x;
+    if (!(ax < 65520.0)) return std::copysign(INFINITY, x);
+    if (ax < 0x1p-14) return std::nearbyint(x * 0x1p24) * 0x1p-24;
+    int e;
+    std::frexp(ax, &e);                      // ax in [2^(e-1), 2^e)
+    const double q = std::ldexp(1.0, e - 11);  // 10 fraction bits
+    return std::nearbyint(x / q) * q;
+}
+
+static void q16_all(double *v, size_t n) {
+    for (size_t i = 0; i < n; ++i) v[i] = oracle_q16(v[i]);
+}
+
 /* PreprocessAtmo.cpp:273-354: T, dE1, dS1, E=0, S=dS1, then for each order
- * n = 2..orders: dJn, dEn, dSn, E += dEn, S += dSn/phaseR. */
+ * n = 2..orders: dJn, dEn, dSn, E += dEn, S += dSn/phaseR.  With
+ * ATMO_FLAG_HALF_STORAGE every table is rounded to FP16 after the pass that
+ * writes it (the reference's RGB16F / RGBA16F textures, PreprocessAtmo.cpp:
+ * 130-143) and the two ADD blends round FP16(dst + src) (:336, :350). */
 void oracle_precompute(const atmo_params *a, const atmo_options *o, double *T, double *E,
                        double *S) {
     P p = mk(a, o);
     size_t n4 = (size_t)p.RR * p.RMU * p.RMUS * p.RNU;
     size_t nE = (size_t)p.SW * p.SH;
+    const bool h = o && (o->flags & ATMO_FLAG_HALF_STORAGE);
+    const size_t nT = (size_t)p.TW * p.TH * 3;
     std::vector<double> dE(nE * 3), dSR(n4 * 3), dSM(n4 * 3), dJ(n4 * 3);
     oracle_pass_transmittance(a, o, T);
+    if (h) q16_all(T, nT);
     oracle_pass_irradiance1(a, o, T, dE.data());
+    if (h) q16_all(dE.data(), dE.size());
     oracle_pass_inscatter1(a, o, T, dSR.data(), dSM.data());
+    if (h) {
+        q16_all(dSR.data(), dSR.size());
+        q16_all(dSM.data(), dSM.size());
+    }
     oracle_pass_copy_irradiance(a, 0.0, dE.data(), E);
     oracle_pass_copy_inscatter1(a, dSR.data(), dSM.data(), S);
     for (int order = 2; order <= p.orders; ++order) {
         int first = order == 2;
         oracle_pass_inscatterS(a, o, T, dE.data(), dSR.data(), dSM.data(), first, dJ.data());
+        if (h) q16_all(dJ.data(), dJ.size());
         oracle_pass_irradianceN(a, o, dSR.data(), dSM.data(), first, dE.data());
+        if (h) q16_all(dE.data(), dE.size());
         oracle_pass_inscatterN(a, o, T, dJ.data(), dSR.data());
+        if (h) q16_all(dSR.data(), dSR.size());
         oracle_pass_copy_irradiance(a, 1.0, dE.data(), E);
+        if (h) q16_all(E, nE * 3);
         oracle_pass_copy_inscatterN(a, dSR.data(), S);
+        if (h) q16_all(S, n4 * 4);
     }
 }
 

@@ -10,7 +10,8 @@
  * src/atmo/sources/proland/preprocess/atmo/ together with an emulation
  * of the OpenGL texture sampling those shaders rely on (GL 3.3 core spec
  * section 3.8.11 LINEAR filtering, CLAMP_TO_EDGE wrap).  Tables are stored
- * in FP64 with no FP16 quantisation (the reference stores RGB16F/RGBA16F).
+ * in FP64; with ATMO_FLAG_HALF_STORAGE, oracle_precompute rounds every stored
+ * table to FP16 as the reference's RGB16F/RGBA16F textures do.
  *
  * Table layouts (doubles, rows as in the reference's .raw files):
  *   T   [TRANSMITTANCE_H][TRANSMITTANCE_W][3]
@@ -75,6 +76,9 @@ ORACLE_API void oracle_pass_inscatterN(const atmo_params *p, const atmo_options 
 ORACLE_API void oracle_pass_copy_inscatterN(const atmo_params *p, const double *dS, double *S);
 ORACLE_API void oracle_pass_copy_irradiance(const atmo_params *p, double k, const double *dE,
                                             double *E);
+
+/* Nearest FP16 value of x (round to nearest even, gradual underflow). */
+ORACLE_API double oracle_q16(double x);
 
 /* Whole precompute (PreprocessAtmo.cpp:273-354 pass order).  T, E, S as above. */
 ORACLE_API void oracle_precompute(const atmo_params *p, const atmo_options *o, double *T,

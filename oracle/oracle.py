@@ -63,6 +63,7 @@ def lib():
             "oracle_pass_copy_inscatterN": (None, [_pp, _dp, _dp]),
             "oracle_pass_copy_irradiance": (None, [_pp, ctypes.c_double, _dp, _dp]),
             "oracle_precompute": (None, [_pp, _po, _dp, _dp, _dp]),
+            "oracle_q16": (ctypes.c_double, [ctypes.c_double]),
             "oracle_sky_radiance": (None, [_pp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp]),
             "oracle_in_scattering": (None, [_pp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double,
                                             ctypes.c_double, _dp, _dp]),
@@ -165,6 +166,13 @@ def precompute(p, o=None):
     T, E, S = zeros_T(p), zeros_E(p), zeros_4D(p, 4)
     lib().oracle_precompute(p, o, _d(T), _d(E), _d(S))
     return T, E, S
+
+
+def q16(x):
+    """Nearest FP16 value of each element (oracle_q16), as float64."""
+    a = np.asarray(x, dtype=np.float64)
+    f = lib().oracle_q16
+    return np.array([f(float(v)) for v in a.ravel()]).reshape(a.shape)
 
 
 def precompute_stages(p, o=None):

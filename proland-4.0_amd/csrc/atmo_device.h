@@ -49,6 +49,7 @@ struct SetDesc {
     // tables (device pointers)
     float4 *T4, *dE4, *E4, *dSR4, *dSM4, *dJ4, *S4;
     int dJb;                        // bytes per dJ texel: 16 (float4) or 12 (packed rgb)
+    int hs;                         // 1: table values rounded to FP16 on store (q16)
     float *outT, *outE;
     const LayerGeom *layers;
 };
@@ -76,6 +77,22 @@ __device__ __forceinline__ void gl_axis_d(double x, int N, int &i0, int &i1, flo
 }
 
 __device__ __forceinline__ float3 f3(float4 v) { return make_float3(v.x, v.y, v.z); }
+
+// Nearest IEEE binary16 value of x, as a float: round to nearest even on the
+// 10 fraction bits for |x| >= 2^-14, on the subnormal quantum 2^-24 below,
+// +-inf from 65520 on (the FP16 overflow threshold).  Integer and exact
+// power-of-two steps only, so it is bit-reproducible.
+__device__ __forceinline__ float q16(float x) {
+    const float ax = fabsf(x);
+    if (!(ax < 65520.0f)) return ax == ax ? copysignf(__builtin_huge_valf(), x) : x;
+    if (ax < 6.103515625e-05f) return rintf(x * 16777216.0f) * 5.9604644775390625e-08f;
+    unsigned b = __float_as_uint(x);
+    b += 0x0FFFu + ((b >> 13) & 1u);
+    return __uint_as_float(b & 0xFFFFE000u);
+}
+__device__ __forceinline__ float4 q16(float4 v, bool on) {
+    return on ? make_float4(q16(v.x), q16(v.y), q16(v.z), q16(v.w)) : v;
+}
 
 // Table pointers reach the kernels through the set descriptors, so the
 // compiler sees generic (flat) pointers; this re-declares them as global

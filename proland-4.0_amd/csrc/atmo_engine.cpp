@@ -201,7 +201,8 @@ bool same_opts(const atmo_options &a, const atmo_options &b) {
     return a.orders == b.orders && a.transmittance_samples == b.transmittance_samples &&
            a.inscatter_samples == b.inscatter_samples && a.irradiance_samples == b.irradiance_samples &&
            a.spherical_samples == b.spherical_samples && a.texels_per_thread == b.texels_per_thread &&
-           a.inscatterN_form == b.inscatterN_form;
+           a.inscatterN_form == b.inscatterN_form &&
+           (a.flags & ATMO_FLAG_HALF_STORAGE) == (b.flags & ATMO_FLAG_HALF_STORAGE);
 }
 
 struct DevBuf {
@@ -735,6 +736,7 @@ atmo_status build_plan(atmo_engine *e, const atmo_params *sets, int nsets, const
         s.dSM4 = (float4 *)b; b += align256(z.n4 * 16);
         s.dJ4 = o.orders >= 2 ? (float4 *)b : nullptr;
         s.dJb = 12;  // the ray gathers of dSn read 12 bytes per texel
+        s.hs = (o.flags & ATMO_FLAG_HALF_STORAGE) ? 1 : 0;
         s.layers = (const LayerGeom *)pl->dlayers.p + loff[i];
     }
     HIPCHK(pl->dsets.ensure(nsets * sizeof(SetDesc)));
@@ -1005,6 +1007,7 @@ static atmo_status pass_setup(atmo_engine *e, const atmo_params *p, const atmo_o
     if ((st = validate(*p, 0)) != ATMO_OK) return st;
     HIPCHK(hipSetDevice(e->device));
     fill_desc(s, *p);
+    s.hs = (oo.flags & ATMO_FLAG_HALF_STORAGE) ? 1 : 0;
     std::vector<LayerGeom> layers;
     for (int l = 0; l < p->RES_R; ++l) layers.push_back(layer_geom(*p, l));
     HIPCHK(e->pass_layers.ensure(layers.size() * sizeof(LayerGeom)));

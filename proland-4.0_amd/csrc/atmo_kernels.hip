@@ -52,6 +52,7 @@ __global__ __launch_bounds__(256) void k_transmittance(const SetDesc *__restrict
         o.y = (float)exp(-((double)s.bR[1] * dR + (double)s.bME[1] * dM));
         o.z = (float)exp(-((double)s.bR[2] * dR + (double)s.bME[2] * dM));
         o.w = 0.0f;
+        o = q16(o, s.hs);
         s.T4[t] = o;
         if (s.outT) {
             s.outT[3 * t + 0] = o.x;
@@ -77,7 +78,7 @@ __global__ __launch_bounds__(256) void k_irradiance1(const SetDesc *__restrict__
     double muS = -0.2 + (double)i / (s.SW - 1.0) * (1.0 + 0.2);
     float3 T = T_lookup(s, (float)h, (float)muS);
     float c = (float)fmax(muS, 0.0);
-    if (s.dE4) s.dE4[t] = make_float4(T.x * c, T.y * c, T.z * c, 0.0f);
+    if (s.dE4) s.dE4[t] = q16(make_float4(T.x * c, T.y * c, T.z * c, 0.0f), s.hs);
     if (s.E4) s.E4[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (s.outE) {
         s.outE[3 * t + 0] = 0.0f;
@@ -234,8 +235,8 @@ __global__ __launch_bounds__(256) void k_inscatter1(const SetDesc *__restrict__ 
         const int row = e / s.W4, x = e % s.W4;
         const size_t t = ((size_t)l * s.RMU + yb + row) * s.W4 + x;
         const float fdx = rowdx[row];
-        const float4 oR = make_float4(aR[j][0] * fdx * s.bR[0], aR[j][1] * fdx * s.bR[1], aR[j][2] * fdx * s.bR[2], 0.f);
-        const float4 oM = make_float4(aM[j][0] * fdx * s.bMS[0], aM[j][1] * fdx * s.bMS[1], aM[j][2] * fdx * s.bMS[2], 0.f);
+        const float4 oR = q16(make_float4(aR[j][0] * fdx * s.bR[0], aR[j][1] * fdx * s.bR[1], aR[j][2] * fdx * s.bR[2], 0.f), s.hs);
+        const float4 oM = q16(make_float4(aM[j][0] * fdx * s.bMS[0], aM[j][1] * fdx * s.bMS[1], aM[j][2] * fdx * s.bMS[2], 0.f), s.hs);
         s.dSR4[t] = oR;
         s.dSM4[t] = oM;
         if (s.S4) s.S4[t] = make_float4(oR.x, oR.y, oR.z, oM.x);
@@ -559,6 +560,9 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
             const int k = kb + g;
             if (k < NU) {
                 const size_t o = ((size_t)l * s.RMU + y) * s.W4 + (size_t)k * s.RMUS + m;
+                if (s.hs) {
+                    acc[g][0] = q16(acc[g][0]); acc[g][1] = q16(acc[g][1]); acc[g][2] = q16(acc[g][2]);
+                }
                 if (s.dJb == 12) {
                     float *q = (float *)s.dJ4 + 3 * o;
                     q[0] = acc[g][0]; q[1] = acc[g][1]; q[2] = acc[g][2];
@@ -677,10 +681,14 @@ __global__ __launch_bounds__(256) void k_irradianceN(const SetDesc *__restrict__
             s1 += red[nw + w];
             s2 += red[2 * nw + w];
         }
+        if (s.hs) {   // dE stored as FP16, then the FP16 ADD blend E = FP16(E + dE)
+            s0 = q16(s0); s1 = q16(s1); s2 = q16(s2);
+        }
         s.dE4[t] = make_float4(s0, s1, s2, 1.0f);
         if (s.E4) {
             float4 e = s.E4[t];
             e.x += s0; e.y += s1; e.z += s2;
+            e = q16(e, s.hs);
             s.E4[t] = e;
             if (s.outE) {
                 s.outE[3 * t + 0] = e.x;
@@ -951,7 +959,7 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
         const int row = e / s.W4, x = e % s.W4;
         const size_t t = ((size_t)l * s.RMU + yb + row) * s.W4 + x;
         const float dx = rowdx[row];
-        const float4 o = make_float4(acc[j][0] * dx, acc[j][1] * dx, acc[j][2] * dx, 0.f);
+        const float4 o = q16(make_float4(acc[j][0] * dx, acc[j][1] * dx, acc[j][2] * dx, 0.f), s.hs);
         s.dSR4[t] = o;
         if (s.S4) {
             const double nu = nuf[j];
@@ -960,7 +968,7 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
             S.x += o.x / pr;
             S.y += o.y / pr;
             S.z += o.z / pr;
-            s.S4[t] = S;
+            s.S4[t] = q16(S, s.hs);   // FP16 ADD blend: S = FP16(S + dS / pR)
         }
     }
 }

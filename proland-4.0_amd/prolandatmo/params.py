@@ -72,6 +72,7 @@ FLAG_CHECK_NUMERICS = 1
 FLAG_NO_GRAPH = 2
 FLAG_TIME_PASSES = 4
 FLAG_NO_AUTOTUNE = 8
+FLAG_HALF_STORAGE = 16
 
 
 def earth(**over):

@@ -16,8 +16,8 @@ torch = pytest.importorskip("torch")
 from oracle import oracle as O  # noqa: E402
 from prolandatmo import lib as L  # noqa: E402
 from prolandatmo.metrics import rel_err, summary  # noqa: E402
-from prolandatmo.params import (AtmoParams, SEED_BATCH, draw_sets, earth, options,  # noqa: E402
-                                with_res)
+from prolandatmo.params import (AtmoParams, FLAG_HALF_STORAGE, SEED_BATCH, draw_sets,  # noqa: E402
+                                earth, options, with_res)
 
 pytestmark = pytest.mark.gpu
 
@@ -142,6 +142,50 @@ def test_chain_matches_oracle(eng, ci):
     _assert("T", T, oT, TOL)
     _assert("E", E, oE, TOL)
     _assert("S", S, oS, TOL)
+
+
+@pytest.mark.parametrize("ci", [0, 2, 3])
+def test_half_storage_matches_oracle(eng, ci):
+    """ATMO_FLAG_HALF_STORAGE (the reference's RGB16F / RGBA16F tables and FP16
+    ADD blending) against the oracle's half-storage chain.  Every GPU value is
+    an FP16 value; a value can differ where the FP32 result before a rounding
+    and the FP64 one fall on either side of a rounding boundary.  Most such
+    flips move a table by one FP16 step (up to 9.8e-4 of the value), but most
+    dJ values are FP16 subnormals (steps of 6e-8 on values of 1e-7 .. 6e-5),
+    and a flipped one moves the ray integrals that gather it by up to a few
+    percent.  Bar: T and E bit-equal on at least 99% of the values and within
+    4 FP16 steps; S bit-equal on at least 99% of the values and within 1e-2
+    (relative, floored at 1e-4 of the table maximum)."""
+    p = CASES[ci]
+    o = options(flags=FLAG_HALF_STORAGE)
+    T, E, S = _chain(eng, p, o)
+    oT, oE, oS = O.precompute(p, o)
+    for name, x, y in (("T16", T, oT), ("E16", E, oE), ("S16", S, oS)):
+        assert np.array_equal(x, x.astype(np.float16).astype(np.float64)), name
+        eq = float(np.mean(x == y))
+        ulp = np.spacing(np.abs(y).astype(np.float16)).astype(np.float64)
+        steps = float(np.max(np.abs(x - y) / ulp))
+        sm = summary(x, y)
+        print("%s bit-equal %.4f, max %.1f FP16 steps, floored rel err max %.3e mean %.3e"
+              % (name, eq, steps, sm["max"], sm["mean"]), flush=True)
+        assert eq >= 0.99, (name, eq)
+        if name == "S16":
+            assert sm["max"] <= 1e-2, (name, sm)
+        else:
+            assert steps <= 4.0, (name, steps)
+
+
+def test_half_storage_is_part_of_the_plan(eng):
+    """The same parameters alternately with and without FP16 storage: each call
+    gets the tables of its own mode (the cached plan and graph are keyed on it)."""
+    p = small()
+    a = _chain(eng, p)
+    b = _chain(eng, p, options(flags=FLAG_HALF_STORAGE))
+    c = _chain(eng, p)
+    for x, y, z in zip(a, b, c):
+        assert np.array_equal(x, z)
+        assert np.array_equal(y, y.astype(np.float16).astype(np.float64))
+    assert not np.array_equal(a[2], b[2])
 
 
 def test_chain_orders_1_and_6(eng):
