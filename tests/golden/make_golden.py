@@ -18,6 +18,11 @@ come from the pinned FP64 oracle (oracle/, checked in tests/test_oracle_pins.py)
                       texels (a seeded subset of a seeded 2 M-texel sample).
                       About 2.5 h of the oracle on 7 threads: run it alone,
                       `python tests/golden/make_golden.py precision_hires`.
+  earth_ref_half.npz  the reference configuration with ATMO_FLAG_HALF_STORAGE (every
+                      table rounded to FP16 as the reference's RGB16F / RGBA16F
+                      textures): T and E in full, S at the 4096 texels of
+                      earth_ref.npz (`... make_golden.py half_storage`, which also
+                      prints how far the FP16-storage tables are from the FP64 ones)
 """
 import os
 import sys
@@ -96,6 +101,32 @@ def precision_hires():
     flat = S.reshape(-1, 4)
     idx = hires_sample(flat.shape[0])
     write_hires(p, T, E, idx, flat[idx], np.abs(S).max(axis=(0, 1, 2)))
+
+
+def half_storage():
+    p = earth()
+    t = time.time()
+    T, E, S = O.precompute(p, options(flags=16))   # ATMO_FLAG_HALF_STORAGE
+    print("earth reference config oracle, FP16 storage: %.1f s on %d threads"
+          % (time.time() - t, O.get_threads()))
+    g = np.load(os.path.join(HERE, "earth_ref.npz"))
+    idx = g["S_idx"]
+    flat = S.reshape(-1, 4)
+    np.savez_compressed(os.path.join(HERE, "earth_ref_half.npz"), params=pvec(p), T=T.astype(np.float32),
+                        E=E.astype(np.float32), S_idx=idx, S_val=flat[idx].astype(np.float32))
+    T64, E64, S64 = O.precompute(p, options())
+
+    def line(name, x, y):
+        fl = 1e-4 * np.abs(y).max()
+        e = np.abs(x - y) / np.maximum(np.abs(y), fl)
+        print("%-4s max %.3e  mean %.3e  >1e-3 %5.1f%%  >1e-2 %5.1f%%"
+              % (name, e.max(), e.mean(), 100 * (e > 1e-3).mean(), 100 * (e > 1e-2).mean()))
+
+    print("FP16-storage tables against the FP64 tables (relative error floored at 1e-4 of the max):")
+    line("T", T, T64)
+    line("E", E, E64)
+    for c in range(4):
+        line("S." + "rgba"[c], S[..., c], S64[..., c])
 
 
 if __name__ == "__main__":

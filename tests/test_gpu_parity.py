@@ -253,6 +253,32 @@ def test_reference_configuration_against_golden_samples(eng):
     assert err.max() <= TOL
 
 
+def test_reference_configuration_half_storage_against_golden_samples(eng):
+    """The reference configuration with ATMO_FLAG_HALF_STORAGE against the
+    oracle's FP16-storage tables: T and E in full, S at 4096 seeded texels.
+    Bar as in test_half_storage_matches_oracle: at least 99% of the values
+    bit-equal, T and E within 4 FP16 steps, S within 1e-2 (floored)."""
+    g = np.load(os.path.join(GOLDEN, "earth_ref_half.npz"))
+    p = _params(g["params"])
+    assert bytes(p) == bytes(earth())
+    T, E, S = _chain(eng, p, options(flags=FLAG_HALF_STORAGE))
+    flat = S.reshape(-1, 4)[g["S_idx"]]
+    for name, x, y in (("T16", T, g["T"].astype(np.float64)), ("E16", E, g["E"].astype(np.float64)),
+                       ("S16", flat, g["S_val"].astype(np.float64))):
+        eq = float(np.mean(x == y))
+        ulp = np.spacing(np.abs(y).astype(np.float16)).astype(np.float64)
+        steps = float(np.max(np.abs(x - y) / ulp))
+        scale = np.abs(S).max() if name == "S16" else np.abs(y).max()
+        err = np.abs(x - y) / np.maximum(np.abs(y), 1e-4 * scale)
+        print("%s bit-equal %.4f, max %.1f FP16 steps, floored rel err max %.3e"
+              % (name, eq, steps, err.max()), flush=True)
+        assert eq >= 0.99, (name, eq)
+        if name == "S16":
+            assert err.max() <= 1e-2
+        else:
+            assert steps <= 4.0, (name, steps)
+
+
 # ---------------------------------------------------------------- batching and graphs
 def test_mixed_resolution_batch_equals_single_runs(eng):
     sets = seeded(3, None)
