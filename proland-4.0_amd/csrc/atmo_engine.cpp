@@ -275,6 +275,7 @@ struct Plan {
     DevBuf dsets, dlayers, dtiles, ddirs, dflags, arena;
     size_t dirJT_off = 0, dirJW_off = 0, dirE_off = 0;
     size_t ldsJ1 = 0, ldsJ = 0, ldsE1 = 0, ldsEN = 0;
+    int chJ1 = 0, chJ = 0, chE1 = 0, chEN = 0;   // theta rows per LDS staging round
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     // kernel variants (picked by the autotuner)
@@ -481,11 +482,12 @@ hipError_t launch_dj(const Plan &pl, const SetDesc *S, const uint2 *T, const cha
                      hipStream_t st) {
     const atmo_options &o = pl.opts;
     const size_t lds = first ? pl.ldsJ1 : pl.ldsJ;
+    const int ch = first ? pl.chJ1 : pl.chJ;
     if (G != 0)
-        return launch_inscatterS(S, T + pl.off[P_J], pl.cnt[P_J], o.spherical_samples, dirs + pl.dirJT_off,
-                                 dirs + pl.dirJW_off, first, lds, G, st);
+        return launch_inscatterS(S, T + pl.off[P_J], pl.cnt[P_J], o.spherical_samples, ch,
+                                 dirs + pl.dirJT_off, dirs + pl.dirJW_off, first, lds, G, st);
     for (const JSeg &g : pl.jseg) {
-        hipError_t e = launch_inscatterS(S, T + pl.off[P_J] + g.off, g.cnt, o.spherical_samples,
+        hipError_t e = launch_inscatterS(S, T + pl.off[P_J] + g.off, g.cnt, o.spherical_samples, ch,
                                          dirs + pl.dirJT_off, dirs + pl.dirJW_off, first, lds, g.G, st);
         if (e != hipSuccess) return e;
     }
@@ -543,7 +545,8 @@ atmo_status launch_all(Plan &pl, hipStream_t st, PassTimer *tm = nullptr) {
         HIPCHK(t.end());
         HIPCHK(t.begin(4, order));
         HIPCHK(launch_irradianceN(S, T + pl.off[P_EN], pl.cnt[P_EN], o.irradiance_samples,
-                                  dirs + pl.dirE_off, first, first ? pl.ldsE1 : pl.ldsEN, st));
+                                  first ? pl.chE1 : pl.chEN, dirs + pl.dirE_off, first,
+                                  first ? pl.ldsE1 : pl.ldsEN, st));
         HIPCHK(t.end());
         HIPCHK(t.begin(5, order));
         HIPCHK(launch_inscatterN(S, T + pl.off[P_SN], pl.cnt[P_SN], o.inscatter_samples, pl.snCap, pl.snSb,
@@ -753,13 +756,16 @@ atmo_status build_plan(atmo_engine *e, const atmo_params *sets, int nsets, const
         maxNU = std::max(maxNU, sets[i].RES_NU);
         maxSW = std::max(maxSW, sets[i].SKY_W);
     }
-    pl->ldsJ1 = lds_inscatterS(o.spherical_samples, maxNU, maxSW, true);
-    pl->ldsJ = lds_inscatterS(o.spherical_samples, maxNU, maxSW, false);
-    pl->ldsE1 = lds_irradianceN(o.irradiance_samples, maxNU, true);
-    pl->ldsEN = lds_irradianceN(o.irradiance_samples, maxNU, false);
-    if (pl->ldsJ1 > 160 * 1024 || pl->ldsE1 > 160 * 1024)
-        return fail(ATMO_ERR_UNSUPPORTED, "spherical staging needs %zu bytes of LDS (> 160 KiB)",
-                    std::max(pl->ldsJ1, pl->ldsE1));
+    pl->chJ1 = chunk_inscatterS(o.spherical_samples, maxNU, maxSW, true);
+    pl->chJ = chunk_inscatterS(o.spherical_samples, maxNU, maxSW, false);
+    pl->chE1 = chunk_irradianceN(o.irradiance_samples, maxNU, true);
+    pl->chEN = chunk_irradianceN(o.irradiance_samples, maxNU, false);
+    if (!pl->chJ1 || !pl->chJ || !pl->chE1 || !pl->chEN)
+        return fail(ATMO_ERR_UNSUPPORTED, "SKY_W %d: the ground irradiance row does not fit in LDS", maxSW);
+    pl->ldsJ1 = lds_inscatterS(pl->chJ1, maxNU, maxSW, true);
+    pl->ldsJ = lds_inscatterS(pl->chJ, maxNU, maxSW, false);
+    pl->ldsE1 = lds_irradianceN(pl->chE1, maxNU, true);
+    pl->ldsEN = lds_irradianceN(pl->chEN, maxNU, false);
     (void)e;
     out = std::move(pl);
     return ATMO_OK;
@@ -1041,15 +1047,18 @@ static atmo_status pass_run(atmo_engine *e, const atmo_params *p, const atmo_opt
     case P_E1: HIPCHK(launch_irradiance1(S, T + off[P_E1], cnt[P_E1], hs)); break;
     case P_S1: HIPCHK(launch_inscatter1(S, T + off[P_S1], cnt[P_S1], o.inscatter_samples, hs)); break;
     case P_J: {
-        size_t lds = lds_inscatterS(o.spherical_samples, p->RES_NU, p->SKY_W, first);
-        if (lds > 160 * 1024) return fail(ATMO_ERR_UNSUPPORTED, "LDS staging too large");
-        HIPCHK(launch_inscatterS(S, T + off[P_J], cnt[P_J], o.spherical_samples, d + oJT,
-                                 d + oJW, first, lds, o.texels_per_thread ? o.texels_per_thread : jg_for_set(*p), hs));
+        const int ch = chunk_inscatterS(o.spherical_samples, p->RES_NU, p->SKY_W, first);
+        if (!ch) return fail(ATMO_ERR_UNSUPPORTED, "SKY_W %d: the ground irradiance row does not fit in LDS", p->SKY_W);
+        const size_t lds = lds_inscatterS(ch, p->RES_NU, p->SKY_W, first);
+        HIPCHK(launch_inscatterS(S, T + off[P_J], cnt[P_J], o.spherical_samples, ch, d + oJT, d + oJW, first,
+                                 lds, o.texels_per_thread ? o.texels_per_thread : jg_for_set(*p), hs));
         break;
     }
     case P_EN: {
-        size_t lds = lds_irradianceN(o.irradiance_samples, p->RES_NU, first);
-        HIPCHK(launch_irradianceN(S, T + off[P_EN], cnt[P_EN], o.irradiance_samples, d + oE,
+        const int ch = chunk_irradianceN(o.irradiance_samples, p->RES_NU, first);
+        if (!ch) return fail(ATMO_ERR_UNSUPPORTED, "irradiance staging does not fit in LDS");
+        const size_t lds = lds_irradianceN(ch, p->RES_NU, first);
+        HIPCHK(launch_irradianceN(S, T + off[P_EN], cnt[P_EN], o.irradiance_samples, ch, d + oE,
                                   first, lds, hs));
         break;
     }

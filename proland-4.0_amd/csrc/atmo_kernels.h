@@ -16,19 +16,26 @@ struct DirEHost {
     float wx, wz, wzdw, pad;
 };
 
-size_t lds_inscatterS(int NS, int RNU, int SW, bool first);
-size_t lds_irradianceN(int NI, int RNU, bool first);
+// The two spherical-integral kernels stage their theta rows in LDS, CH rows at
+// a time: CH is the largest count (up to all of them) whose staging fits
+// kSphLdsBudget, the dynamic LDS a launch gets without raising the kernel's
+// limit; 0 = not even one row fits.  lds_* are the bytes for CH rows.
+constexpr size_t kSphLdsBudget = 64 * 1024;
+size_t lds_inscatterS(int CH, int RNU, int SW, bool first);
+size_t lds_irradianceN(int CH, int RNU, bool first);
+int chunk_inscatterS(int NS, int RNU, int SW, bool first);
+int chunk_irradianceN(int NI, int RNU, bool first);
 
 hipError_t launch_transmittance(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
                                 hipStream_t st);
 hipError_t launch_irradiance1(const SetDesc *sets, const uint2 *tiles, int ntiles, hipStream_t st);
 hipError_t launch_inscatter1(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
                              hipStream_t st);
-// G: texels per thread (2, 3, 4 or 8)
-hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS,
+// G: texels per thread (2, 3, 4 or 8); CH: theta rows per staging round
+hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS, int CH,
                              const void *dirT, const void *dirW, bool first, size_t lds, int G,
                              hipStream_t st);
-hipError_t launch_irradianceN(const SetDesc *sets, const uint2 *tiles, int ntiles, int NI,
+hipError_t launch_irradianceN(const SetDesc *sets, const uint2 *tiles, int ntiles, int NI, int CH,
                               const void *dirs, bool first, size_t lds, hipStream_t st);
 // rpw_cap: at most this many mu rows per workgroup (1, 2 or 4); the tile table
 // must have been built with the same cap.  sb = 0: direct gathers; sb > 0:

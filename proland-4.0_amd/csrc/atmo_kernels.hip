@@ -303,9 +303,14 @@ struct DirJ {      // per-theta constants of the NS x 2NS grid
     float ct, st, dw, pad;
 };
 
-template <bool FIRST, int G>
+// CHUNK: the theta rows are staged CH at a time (when the staging of all NS
+// rows exceeds the LDS budget); between chunks a thread keeps its running sums
+// in the dJ table itself and resumes them, so the sum over theta runs in the
+// same order, and gives the same bits, as with all rows staged at once
+// (!CHUNK, CH = NS: one round, compiled without the chunk bookkeeping).
+template <bool FIRST, int G, bool CHUNK>
 __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ sets,
-                                                    const uint2 *__restrict__ tiles, int NS,
+                                                    const uint2 *__restrict__ tiles, int NS, int CH,
                                                     const DirJ *__restrict__ dirT,
                                                     const float2 *__restrict__ dirW) {
     constexpr int NT = FIRST ? 2 : 1;
@@ -318,11 +323,11 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
     const int NU = s.RNU;
     const int NP = NU + 1;                 // pairs per theta
     const int NV = NU + 2;                 // slice values per theta
-    float4 *pairs = lds;                                   // [NS][NP][PW]
-    float4 *vtmp = pairs + (size_t)NS * NP * PW;           // [NS][NV][NT]
+    float4 *pairs = lds;                                   // [CH][NP][PW]
+    float4 *vtmp = pairs + (size_t)CH * NP * PW;           // [CH][NV][NT]
     const int EPAD = e_pad(s.SW);
-    float4 *epair = vtmp + (size_t)NS * NV * NT;           // [EPAD + SW + 1][2]
-    float4 *gdat = epair + (size_t)(EPAD + s.SW + 1) * 2;  // [NS][2]
+    float4 *epair = vtmp + (size_t)CH * NV * NT;           // [EPAD + SW + 1][2]
+    float4 *gdat = epair + (size_t)(EPAD + s.SW + 1) * 2;  // [CH][2]
 
     const LayerGeom L = s.layers[l];
     const double r = fmin(fmax(L.r, s.Rg), s.Rt);          // inscatterS.glsl:112
@@ -332,22 +337,6 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
     const float4 *tabs[2] = {s.dSR4, s.dSM4};
     const double posAd = 0.5 * (NU - 1), posBd = posAd + 1.0;
 
-    // (1) slice values V[it][kk], kk = slice + 1
-    for (int e = threadIdx.x; e < NS * NV; e += blockDim.x) {
-        const int it = e / NV, kk = e % NV;
-        const double ct = cos((it + 0.5) * (kPI / NS));
-        const double rmu = r * ct;
-        double tz, ty;
-        tex4d_rmu(s, rho2, rmu, rmu * rmu - rho2, tz, ty);
-        float4 v[NT];
-        stage_slice<NT>(s, tabs, tz, ty, txs, kk - 1, v);
-        if (FIRST) {  // fold the phase-function constants in
-            v[0].x *= s.kR; v[0].y *= s.kR; v[0].z *= s.kR;
-            v[NT - 1].x *= s.kM; v[NT - 1].y *= s.kM; v[NT - 1].z *= s.kM;
-        }
-#pragma unroll
-        for (int n = 0; n < NT; ++n) vtmp[(size_t)e * NT + n] = v[n];
-    }
     // (2) row 0 of dE (irradiance at r = Rg reads only row 0 under CLAMP_TO_EDGE),
     // with EPAD copies of texel 0 in front and one of texel SW-1 behind (zero
     // differences): every texel coordinate the ground term can produce,
@@ -360,63 +349,6 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
         epair[2 * k] = make_float4(a.x, a.y, a.z, b.x - a.x);
         epair[2 * k + 1] = make_float4(b.y - a.y, b.z - a.z, 0.f, 0.f);
     }
-    // (3) per theta: ground visibility, ground-to-x transmittance (:129-138), and
-    // the theta-uniform parts of the lookup positions
-    for (int it = threadIdx.x; it < NS; it += blockDim.x) {
-        const double ct = cos((it + 0.5) * (kPI / NS));
-        const double q = s.Rg / r;
-        const double cthetamin = -sqrt(1.0 - q * q);
-        float4 g0 = make_float4(0.f, 0.f, 0.f, 0.f);
-        double c1 = 0.0, c0 = 0.0;
-        if (ct < cthetamin) {
-            const double dg = -r * ct - sqrt(r * r * (ct * ct - 1.0) + s.Rg2);
-            const double mug = -(r * ct + dg) / s.Rg;
-            // transmittance(Rg, mug, dg) (common.glsl:272-282); its end point is x itself
-            const double r1sq = s.Rg2 + dg * dg + 2.0 * s.Rg * mug * dg;
-            const double r1 = sqrt(r1sq);
-            const double mu1 = (s.Rg * mug + dg) / r1;
-            const float h1 = (float)((r1sq - s.Rg2) / (r1 + s.Rg));
-            float3 tr;
-            if (mug > 0.0) {
-                float3 a = T_lookup(s, 0.0f, (float)mug), b = T_lookup(s, h1, (float)mu1);
-                tr = make_float3(fminf(a.x / b.x, 1.f), fminf(a.y / b.y, 1.f), fminf(a.z / b.z, 1.f));
-            } else {
-                float3 a = T_lookup(s, h1, (float)-mu1), b = T_lookup(s, 0.0f, (float)-mug);
-                tr = make_float3(fminf(a.x / b.x, 1.f), fminf(a.y / b.y, 1.f), fminf(a.z / b.z, 1.f));
-            }
-            const float gr = (float)(s.alpha / kPI);
-            g0 = make_float4(gr * tr.x, gr * tr.y, gr * tr.z, 1.0f);
-            // E texel coordinate ((r muS + dg nu1)/Rg + 0.2)/1.2 SW - 0.5 as c1 pos + c0
-            const double sc = s.SW / 1.2;
-            const double e1 = dg / s.Rg * sc, e0 = (r * muS / s.Rg + 0.2) * sc - 0.5 + EPAD;
-            c1 = e1 / posAd;
-            c0 = e0 - e1 * posBd / posAd;
-        }
-        gdat[2 * it] = g0;
-        gdat[2 * it + 1] = make_float4((float)c1, (float)c0, (float)(posAd * muS * ct + posBd),
-                                       (float)((kPI / NS) * (kPI / NS) * sin((it + 0.5) * (kPI / NS))));
-    }
-    __syncthreads();
-    // (4) pairs {V[kk-1], V[kk] - V[kk-1]}
-    for (int e = threadIdx.x; e < NS * NP; e += blockDim.x) {
-        const int it = e / NP, kk = e % NP;
-        const float4 a = vtmp[((size_t)it * NV + kk) * NT];
-        const float4 b = vtmp[((size_t)it * NV + kk + 1) * NT];
-        float4 *q = pairs + (size_t)e * PW;
-        if (!FIRST) {
-            // (V.rgb, D.r), (D.g, D.b): one 16-byte and one 8-byte LDS read
-            q[0] = make_float4(a.x, a.y, a.z, b.x - a.x);
-            q[1] = make_float4(b.y - a.y, b.z - a.z, 0.f, 0.f);
-        } else {
-            // Rayleigh and Mie pairs in 48 bytes: three 16-byte LDS reads
-            const float4 am = vtmp[((size_t)it * NV + kk) * NT + 1];
-            const float4 bm = vtmp[((size_t)it * NV + kk + 1) * NT + 1];
-            q[0] = make_float4(a.x, a.y, a.z, b.x - a.x);
-            q[1] = make_float4(b.y - a.y, b.z - a.z, am.x, am.y);
-            q[2] = make_float4(am.z, bm.x - am.x, bm.y - am.y, bm.z - am.z);
-        }
-    }
-    __syncthreads();
 
     // per-layer scattering coefficients (phase constants folded in)
     const double hr = r - s.Rg;
@@ -428,149 +360,247 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
     const int ngk = (NU + G - 1) / G;
     const int ngroups = s.RMU * ngk;
 
-    for (int grp = threadIdx.x; grp < ngroups; grp += blockDim.x) {
-        const int y = grp % s.RMU, kb = (grp / s.RMU) * G;
-        float vx = 0.f, vz = 0.f;
-        float sxp[G], syp[G];
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const int k = min(kb + g, NU - 1);
-            double mu, muS_t, nu;
-            mu_mus_nu(s, L, k * s.RMUS + m, y, mu, muS_t, nu);
-            // inscatterS.glsl:113-122 (FP64)
-            mu = fmin(fmax(mu, -1.0), 1.0);
-            const double var = sqrt(1.0 - mu * mu) * sqrt(1.0 - muS * muS);
-            nu = fmin(fmax(nu, muS * mu - var), muS * mu + var);
-            const double vxd = sqrt(1.0 - mu * mu);
-            const double sxd = vxd == 0.0 ? 0.0 : (nu - muS * mu) / vxd;
-            const double syd = sqrt(fmax(0.0, 1.0 - sxd * sxd - muS * muS));
-            vx = (float)vxd;
-            vz = (float)mu;
-            sxp[g] = (float)(posAd * sxd);
-            syp[g] = (float)(posAd * syd);
-        }
-        const float cx1 = m2g * vx;
-        float acc[G][3];
-#pragma unroll
-        for (int g = 0; g < G; ++g)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) acc[g][c] = 0.f;
-
-        // 4D lookup of one direction from the staged pairs: the clamp of the
-        // nu slice is implicit, |s.w| <= 1 keeps pos in [1, RNU] (+ rounding)
-        // and kk = trunc(pos) in [0, RNU] = the staged pair range.
-        // (order 2: the Rayleigh and Mie values weighted by the phase functions of
-        // nu1 = s.w, as q1 (R + z1 M) with q1 = 1 + nu1^2, z1 = (1+g^2-2g nu1)^-3/2;
-        // the caller applies q1, whose per-set constants are folded into the staging)
-        auto lookup = [&](const float4 *pt, float pos, float out[3], float &q1) {
-            const unsigned kk = __float2uint_rz(pos);
-            const float f = __builtin_amdgcn_fractf(pos);
-            const float4 *pp = pt + kk * PW;
-            const float4 a = pp[0];
-            if (!FIRST) {
-                const float2 d = *(const float2 *)(pp + 1);
-                out[0] = fmaf(f, a.w, a.x);
-                out[1] = fmaf(f, d.x, a.y);
-                out[2] = fmaf(f, d.y, a.z);
-                q1 = 1.0f;
-            } else {
-                const float4 b = pp[1], c2 = pp[2];
-                const float nu1 = fmaf(pos, iposA, nposBi);
-                q1 = fmaf(nu1, nu1, 1.0f);
-                const float y1 = __builtin_amdgcn_rsqf(fmaf(m2g, nu1, g2p1));
-                const float z1 = y1 * y1 * y1;
-                const float m0 = fmaf(f, c2.y, b.z), m1 = fmaf(f, c2.z, b.w), m2 = fmaf(f, c2.w, c2.x);
-                out[0] = fmaf(m0, z1, fmaf(f, a.w, a.x));
-                out[1] = fmaf(m1, z1, fmaf(f, b.x, a.y));
-                out[2] = fmaf(m2, z1, fmaf(f, b.y, a.z));
+    for (int c0 = 0; c0 < NS; c0 += CH) {
+        const int nth = CHUNK ? min(CH, NS - c0) : NS;
+        const bool last = !CHUNK || c0 + nth == NS;
+        if (CHUNK && c0 > 0) __syncthreads();   // every thread is done with the previous chunk's rows
+        // (1) slice values V[it][kk], kk = slice + 1
+        for (int e = threadIdx.x; e < nth * NV; e += blockDim.x) {
+            const int it = c0 + e / NV, kk = e % NV;
+            const double ct = cos((it + 0.5) * (kPI / NS));
+            const double rmu = r * ct;
+            double tz, ty;
+            tex4d_rmu(s, rho2, rmu, rmu * rmu - rho2, tz, ty);
+            float4 v[NT];
+            stage_slice<NT>(s, tabs, tz, ty, txs, kk - 1, v);
+            if (FIRST) {  // fold the phase-function constants in
+                v[0].x *= s.kR; v[0].y *= s.kR; v[0].z *= s.kR;
+                v[NT - 1].x *= s.kM; v[NT - 1].y *= s.kM; v[NT - 1].z *= s.kM;
             }
-        };
-        // irradiance at the ground point seen in direction w, row 0 of dE
-        auto ground_e = [&](float pos, float cg1, float cg0, float out[3]) {
-            const float xg = fmaf(pos, cg1, cg0);   // >= 0: EPAD is folded into cg0
-            const unsigned kg = __float2uint_rz(xg);
-            const float fg = __builtin_amdgcn_fractf(xg);
-            const float4 ea = epair[2 * kg];
-            const float2 ed = *(const float2 *)(epair + 2 * kg + 1);
-            out[0] = fmaf(fg, ea.w, ea.x);
-            out[1] = fmaf(fg, ed.x, ea.y);
-            out[2] = fmaf(fg, ed.y, ea.z);
-        };
-
-        for (int it = 0; it < NS; ++it) {
-            const float ct = dirT[it].ct;
-            const float4 g0 = gdat[2 * it], g1 = gdat[2 * it + 1];
-            const bool ground = g0.w != 0.0f;
-            const float vzwz = vz * ct;
-            const float cx0 = fmaf(m2g, vzwz, g2p1);
-            const float dw = g1.w, cp = g1.z;
-            const float4 *pt = pairs + (size_t)it * NP * PW;
-            const float2 *wrow = dirW + (size_t)it * 2 * NS;
-            float2 wn = wrow[0];
-            // directions phi and 2 pi - phi together: same w.x, opposite w.y,
-            // hence the same nu2 = v.w (v has no y component) and weights
-            for (int ip = 0; ip < NS; ++ip) {
-                const float2 w = wn;
-                if (ip + 1 < NS) wn = wrow[ip + 1];  // scalar prefetch of the next direction
-                const float nu2 = fmaf(vx, w.x, vzwz);
-                const float q2 = fmaf(nu2 * dw, nu2, dw);
-                const float y2 = __builtin_amdgcn_rsqf(fmaf(cx1, w.x, cx0));
-                const float t2 = q2 * (y2 * y2 * y2);
-                // betaR e^(-h/HR) pR(nu2) dw + betaMSca e^(-h/HM) pM(nu2) dw (:182-185)
-                const float W0 = fmaf(A0, q2, B0 * t2), W1 = fmaf(A1, q2, B1 * t2),
-                            W2 = fmaf(A2, q2, B2 * t2);
-                float pa[G], pb[G], lv[G][3];
-#pragma unroll
-                for (int g = 0; g < G; ++g) {
-                    const float base = fmaf(sxp[g], w.x, cp);
-                    pa[g] = fmaf(syp[g], w.y, base);
-                    pb[g] = fmaf(-syp[g], w.y, base);
-                    float la[3], lb[3], qa, qb;
-                    lookup(pt, pa[g], la, qa);
-                    lookup(pt, pb[g], lb, qb);
-                    if (FIRST) {
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) lv[g][c] = fmaf(qa, la[c], qb * lb[c]);
-                    } else {
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) lv[g][c] = la[c] + lb[c];
-                    }
-                }
-                if (ground) {
-#pragma unroll
-                    for (int g = 0; g < G; ++g) {
-                        float ea[3], eb[3];
-                        ground_e(pa[g], g1.x, g1.y, ea);
-                        ground_e(pb[g], g1.x, g1.y, eb);
-                        lv[g][0] = fmaf(g0.x, ea[0] + eb[0], lv[g][0]);
-                        lv[g][1] = fmaf(g0.y, ea[1] + eb[1], lv[g][1]);
-                        lv[g][2] = fmaf(g0.z, ea[2] + eb[2], lv[g][2]);
-                    }
-                }
-#pragma unroll
-                for (int g = 0; g < G; ++g) {
-                    acc[g][0] = fmaf(lv[g][0], W0, acc[g][0]);
-                    acc[g][1] = fmaf(lv[g][1], W1, acc[g][1]);
-                    acc[g][2] = fmaf(lv[g][2], W2, acc[g][2]);
-                }
-            }
+    #pragma unroll
+            for (int n = 0; n < NT; ++n) vtmp[(size_t)e * NT + n] = v[n];
         }
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const int k = kb + g;
-            if (k < NU) {
-                const size_t o = ((size_t)l * s.RMU + y) * s.W4 + (size_t)k * s.RMUS + m;
-                if (s.hs) {
-                    acc[g][0] = q16(acc[g][0]); acc[g][1] = q16(acc[g][1]); acc[g][2] = q16(acc[g][2]);
-                }
-                if (s.dJb == 12) {
-                    float *q = (float *)s.dJ4 + 3 * o;
-                    q[0] = acc[g][0]; q[1] = acc[g][1]; q[2] = acc[g][2];
+        // (3) per theta: ground visibility, ground-to-x transmittance (:129-138), and
+        // the theta-uniform parts of the lookup positions
+        for (int itl = threadIdx.x; itl < nth; itl += blockDim.x) {
+            const int it = c0 + itl;
+            const double ct = cos((it + 0.5) * (kPI / NS));
+            const double q = s.Rg / r;
+            const double cthetamin = -sqrt(1.0 - q * q);
+            float4 g0 = make_float4(0.f, 0.f, 0.f, 0.f);
+            double c1 = 0.0, ce0 = 0.0;
+            if (ct < cthetamin) {
+                const double dg = -r * ct - sqrt(r * r * (ct * ct - 1.0) + s.Rg2);
+                const double mug = -(r * ct + dg) / s.Rg;
+                // transmittance(Rg, mug, dg) (common.glsl:272-282); its end point is x itself
+                const double r1sq = s.Rg2 + dg * dg + 2.0 * s.Rg * mug * dg;
+                const double r1 = sqrt(r1sq);
+                const double mu1 = (s.Rg * mug + dg) / r1;
+                const float h1 = (float)((r1sq - s.Rg2) / (r1 + s.Rg));
+                float3 tr;
+                if (mug > 0.0) {
+                    float3 a = T_lookup(s, 0.0f, (float)mug), b = T_lookup(s, h1, (float)mu1);
+                    tr = make_float3(fminf(a.x / b.x, 1.f), fminf(a.y / b.y, 1.f), fminf(a.z / b.z, 1.f));
                 } else {
-                    s.dJ4[o] = make_float4(acc[g][0], acc[g][1], acc[g][2], 0.f);
+                    float3 a = T_lookup(s, h1, (float)-mu1), b = T_lookup(s, 0.0f, (float)-mug);
+                    tr = make_float3(fminf(a.x / b.x, 1.f), fminf(a.y / b.y, 1.f), fminf(a.z / b.z, 1.f));
+                }
+                const float gr = (float)(s.alpha / kPI);
+                g0 = make_float4(gr * tr.x, gr * tr.y, gr * tr.z, 1.0f);
+                // E texel coordinate ((r muS + dg nu1)/Rg + 0.2)/1.2 SW - 0.5 as c1 pos + ce0
+                const double sc = s.SW / 1.2;
+                const double e1 = dg / s.Rg * sc, e0 = (r * muS / s.Rg + 0.2) * sc - 0.5 + EPAD;
+                c1 = e1 / posAd;
+                ce0 = e0 - e1 * posBd / posAd;
+            }
+            gdat[2 * itl] = g0;
+            gdat[2 * itl + 1] = make_float4((float)c1, (float)ce0, (float)(posAd * muS * ct + posBd),
+                                            (float)((kPI / NS) * (kPI / NS) * sin((it + 0.5) * (kPI / NS))));
+        }
+        __syncthreads();
+        // (4) pairs {V[kk-1], V[kk] - V[kk-1]}
+        for (int e = threadIdx.x; e < nth * NP; e += blockDim.x) {
+            const int it = e / NP, kk = e % NP;
+            const float4 a = vtmp[((size_t)it * NV + kk) * NT];
+            const float4 b = vtmp[((size_t)it * NV + kk + 1) * NT];
+            float4 *q = pairs + (size_t)e * PW;
+            if (!FIRST) {
+                // (V.rgb, D.r), (D.g, D.b): one 16-byte and one 8-byte LDS read
+                q[0] = make_float4(a.x, a.y, a.z, b.x - a.x);
+                q[1] = make_float4(b.y - a.y, b.z - a.z, 0.f, 0.f);
+            } else {
+                // Rayleigh and Mie pairs in 48 bytes: three 16-byte LDS reads
+                const float4 am = vtmp[((size_t)it * NV + kk) * NT + 1];
+                const float4 bm = vtmp[((size_t)it * NV + kk + 1) * NT + 1];
+                q[0] = make_float4(a.x, a.y, a.z, b.x - a.x);
+                q[1] = make_float4(b.y - a.y, b.z - a.z, am.x, am.y);
+                q[2] = make_float4(am.z, bm.x - am.x, bm.y - am.y, bm.z - am.z);
+            }
+        }
+        __syncthreads();
+
+        for (int grp = threadIdx.x; grp < ngroups; grp += blockDim.x) {
+            const int y = grp % s.RMU, kb = (grp / s.RMU) * G;
+            float vx = 0.f, vz = 0.f;
+            float sxp[G], syp[G];
+    #pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const int k = min(kb + g, NU - 1);
+                double mu, muS_t, nu;
+                mu_mus_nu(s, L, k * s.RMUS + m, y, mu, muS_t, nu);
+                // inscatterS.glsl:113-122 (FP64)
+                mu = fmin(fmax(mu, -1.0), 1.0);
+                const double var = sqrt(1.0 - mu * mu) * sqrt(1.0 - muS * muS);
+                nu = fmin(fmax(nu, muS * mu - var), muS * mu + var);
+                const double vxd = sqrt(1.0 - mu * mu);
+                const double sxd = vxd == 0.0 ? 0.0 : (nu - muS * mu) / vxd;
+                const double syd = sqrt(fmax(0.0, 1.0 - sxd * sxd - muS * muS));
+                vx = (float)vxd;
+                vz = (float)mu;
+                sxp[g] = (float)(posAd * sxd);
+                syp[g] = (float)(posAd * syd);
+            }
+            const float cx1 = m2g * vx;
+            float acc[G][3];
+    #pragma unroll
+            for (int g = 0; g < G; ++g)
+    #pragma unroll
+                for (int c = 0; c < 3; ++c) acc[g][c] = 0.f;
+            if (CHUNK && c0 > 0) {   // resume the sums of the previous chunks (written by this thread)
+    #pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    const int k = kb + g;
+                    if (k < NU) {
+                        const size_t o = ((size_t)l * s.RMU + y) * s.W4 + (size_t)k * s.RMUS + m;
+                        if (s.dJb == 12) {
+                            const float *q = (const float *)s.dJ4 + 3 * o;
+                            acc[g][0] = q[0]; acc[g][1] = q[1]; acc[g][2] = q[2];
+                        } else {
+                            const float4 q = s.dJ4[o];
+                            acc[g][0] = q.x; acc[g][1] = q.y; acc[g][2] = q.z;
+                        }
+                    }
+                }
+            }
+
+            // 4D lookup of one direction from the staged pairs: the clamp of the
+            // nu slice is implicit, |s.w| <= 1 keeps pos in [1, RNU] (+ rounding)
+            // and kk = trunc(pos) in [0, RNU] = the staged pair range.
+            // (order 2: the Rayleigh and Mie values weighted by the phase functions of
+            // nu1 = s.w, as q1 (R + z1 M) with q1 = 1 + nu1^2, z1 = (1+g^2-2g nu1)^-3/2;
+            // the caller applies q1, whose per-set constants are folded into the staging)
+            auto lookup = [&](const float4 *pt, float pos, float out[3], float &q1) {
+                const unsigned kk = __float2uint_rz(pos);
+                const float f = __builtin_amdgcn_fractf(pos);
+                const float4 *pp = pt + kk * PW;
+                const float4 a = pp[0];
+                if (!FIRST) {
+                    const float2 d = *(const float2 *)(pp + 1);
+                    out[0] = fmaf(f, a.w, a.x);
+                    out[1] = fmaf(f, d.x, a.y);
+                    out[2] = fmaf(f, d.y, a.z);
+                    q1 = 1.0f;
+                } else {
+                    const float4 b = pp[1], c2 = pp[2];
+                    const float nu1 = fmaf(pos, iposA, nposBi);
+                    q1 = fmaf(nu1, nu1, 1.0f);
+                    const float y1 = __builtin_amdgcn_rsqf(fmaf(m2g, nu1, g2p1));
+                    const float z1 = y1 * y1 * y1;
+                    const float m0 = fmaf(f, c2.y, b.z), m1 = fmaf(f, c2.z, b.w), m2 = fmaf(f, c2.w, c2.x);
+                    out[0] = fmaf(m0, z1, fmaf(f, a.w, a.x));
+                    out[1] = fmaf(m1, z1, fmaf(f, b.x, a.y));
+                    out[2] = fmaf(m2, z1, fmaf(f, b.y, a.z));
+                }
+            };
+            // irradiance at the ground point seen in direction w, row 0 of dE
+            auto ground_e = [&](float pos, float cg1, float cg0, float out[3]) {
+                const float xg = fmaf(pos, cg1, cg0);   // >= 0: EPAD is folded into cg0
+                const unsigned kg = __float2uint_rz(xg);
+                const float fg = __builtin_amdgcn_fractf(xg);
+                const float4 ea = epair[2 * kg];
+                const float2 ed = *(const float2 *)(epair + 2 * kg + 1);
+                out[0] = fmaf(fg, ea.w, ea.x);
+                out[1] = fmaf(fg, ed.x, ea.y);
+                out[2] = fmaf(fg, ed.y, ea.z);
+            };
+
+            for (int itl = 0; itl < nth; ++itl) {
+                const int it = c0 + itl;
+                const float ct = dirT[it].ct;
+                const float4 g0 = gdat[2 * itl], g1 = gdat[2 * itl + 1];
+                const bool ground = g0.w != 0.0f;
+                const float vzwz = vz * ct;
+                const float cx0 = fmaf(m2g, vzwz, g2p1);
+                const float dw = g1.w, cp = g1.z;
+                const float4 *pt = pairs + (size_t)itl * NP * PW;
+                const float2 *wrow = dirW + (size_t)it * 2 * NS;
+                float2 wn = wrow[0];
+                // directions phi and 2 pi - phi together: same w.x, opposite w.y,
+                // hence the same nu2 = v.w (v has no y component) and weights
+                for (int ip = 0; ip < NS; ++ip) {
+                    const float2 w = wn;
+                    if (ip + 1 < NS) wn = wrow[ip + 1];  // scalar prefetch of the next direction
+                    const float nu2 = fmaf(vx, w.x, vzwz);
+                    const float q2 = fmaf(nu2 * dw, nu2, dw);
+                    const float y2 = __builtin_amdgcn_rsqf(fmaf(cx1, w.x, cx0));
+                    const float t2 = q2 * (y2 * y2 * y2);
+                    // betaR e^(-h/HR) pR(nu2) dw + betaMSca e^(-h/HM) pM(nu2) dw (:182-185)
+                    const float W0 = fmaf(A0, q2, B0 * t2), W1 = fmaf(A1, q2, B1 * t2),
+                                W2 = fmaf(A2, q2, B2 * t2);
+                    float pa[G], pb[G], lv[G][3];
+    #pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const float base = fmaf(sxp[g], w.x, cp);
+                        pa[g] = fmaf(syp[g], w.y, base);
+                        pb[g] = fmaf(-syp[g], w.y, base);
+                        float la[3], lb[3], qa, qb;
+                        lookup(pt, pa[g], la, qa);
+                        lookup(pt, pb[g], lb, qb);
+                        if (FIRST) {
+    #pragma unroll
+                            for (int c = 0; c < 3; ++c) lv[g][c] = fmaf(qa, la[c], qb * lb[c]);
+                        } else {
+    #pragma unroll
+                            for (int c = 0; c < 3; ++c) lv[g][c] = la[c] + lb[c];
+                        }
+                    }
+                    if (ground) {
+    #pragma unroll
+                        for (int g = 0; g < G; ++g) {
+                            float ea[3], eb[3];
+                            ground_e(pa[g], g1.x, g1.y, ea);
+                            ground_e(pb[g], g1.x, g1.y, eb);
+                            lv[g][0] = fmaf(g0.x, ea[0] + eb[0], lv[g][0]);
+                            lv[g][1] = fmaf(g0.y, ea[1] + eb[1], lv[g][1]);
+                            lv[g][2] = fmaf(g0.z, ea[2] + eb[2], lv[g][2]);
+                        }
+                    }
+    #pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        acc[g][0] = fmaf(lv[g][0], W0, acc[g][0]);
+                        acc[g][1] = fmaf(lv[g][1], W1, acc[g][1]);
+                        acc[g][2] = fmaf(lv[g][2], W2, acc[g][2]);
+                    }
+                }
+            }
+    #pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const int k = kb + g;
+                if (k < NU) {
+                    const size_t o = ((size_t)l * s.RMU + y) * s.W4 + (size_t)k * s.RMUS + m;
+                    if (s.hs && last) {
+                        acc[g][0] = q16(acc[g][0]); acc[g][1] = q16(acc[g][1]); acc[g][2] = q16(acc[g][2]);
+                    }
+                    if (s.dJb == 12) {
+                        float *q = (float *)s.dJ4 + 3 * o;
+                        q[0] = acc[g][0]; q[1] = acc[g][1]; q[2] = acc[g][2];
+                    } else {
+                        s.dJ4[o] = make_float4(acc[g][0], acc[g][1], acc[g][2], 0.f);
+                    }
                 }
             }
         }
+        if (!CHUNK) break;
     }
 }
 
@@ -582,9 +612,11 @@ struct DirE {      // per-direction constants of the hemisphere grid
     float wx, wz, wzdw, pad;
 };
 
+// The NI/2 theta rows are staged CH at a time (CH = NI/2 unless they exceed
+// the LDS budget).
 template <bool FIRST>
 __global__ __launch_bounds__(256) void k_irradianceN(const SetDesc *__restrict__ sets,
-                                                     const uint2 *__restrict__ tiles, int NI,
+                                                     const uint2 *__restrict__ tiles, int NI, int CH,
                                                      const DirE *__restrict__ dirs) {
     constexpr int NT = FIRST ? 2 : 1;
     extern __shared__ float4 lds[];
@@ -594,9 +626,9 @@ __global__ __launch_bounds__(256) void k_irradianceN(const SetDesc *__restrict__
     const int i = t % s.SW, j = t / s.SW;
     const int NU = s.RNU, NP = NU + 1, NV = NU + 2;
     const int NTH = NI / 2;
-    float4 *pairs = lds;                                   // [NTH][NP][NT][2]
-    float4 *vtmp = pairs + (size_t)NTH * NP * NT * 2;      // [NTH][NV][NT]
-    float *red = (float *)(vtmp + (size_t)NTH * NV * NT);  // [3][blockDim/64]
+    float4 *pairs = lds;                                   // [CH][NP][NT][2]
+    float4 *vtmp = pairs + (size_t)CH * NP * NT * 2;       // [CH][NV][NT]
+    float *red = (float *)(vtmp + (size_t)CH * NV * NT);   // [3][blockDim/64]
 
     const double h = (double)j / (s.SH - 1.0) * (s.Rt - s.Rg);
     const double r = s.Rg + h;
@@ -604,61 +636,66 @@ __global__ __launch_bounds__(256) void k_irradianceN(const SetDesc *__restrict__
     const double rho2 = h * (2.0 * s.Rg + h);
     const double txs = tex4d_umus(s, muS) * s.RMUS - 0.5;
     const float4 *tabs[2] = {s.dSR4, s.dSM4};
-    for (int e = threadIdx.x; e < NTH * NV; e += blockDim.x) {
-        const int it = e / NV, kk = e % NV;
-        const double ct = cos((it + 0.5) * (kPI / NI));
-        const double rmu = r * ct;
-        double tz, ty;
-        tex4d_rmu(s, rho2, rmu, rmu * rmu - rho2, tz, ty);
-        float4 v[NT];
-        stage_slice<NT>(s, tabs, tz, ty, txs, kk - 1, v);
-        if (FIRST) {
-            v[0].x *= s.kR; v[0].y *= s.kR; v[0].z *= s.kR;
-            v[NT - 1].x *= s.kM; v[NT - 1].y *= s.kM; v[NT - 1].z *= s.kM;
-        }
-#pragma unroll
-        for (int n = 0; n < NT; ++n) vtmp[(size_t)e * NT + n] = v[n];
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < NTH * NP * NT; e += blockDim.x) {
-        const int n = e % NT, ek = e / NT;
-        const int it = ek / NP, kk = ek % NP;
-        float4 a = vtmp[((size_t)it * NV + kk) * NT + n];
-        float4 b = vtmp[((size_t)it * NV + kk + 1) * NT + n];
-        pairs[((size_t)(it * NP + kk) * NT + n) * 2] = a;
-        pairs[((size_t)(it * NP + kk) * NT + n) * 2 + 1] =
-            make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.f);
-    }
-    __syncthreads();
-
     const float sx = (float)sqrt(fmax(1.0 - muS * muS, 0.0)), sz = (float)muS;
     const float posA = 0.5f * (NU - 1), posB = 0.5f * (NU - 1) + 1.0f;
     float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-    const int ndir = 2 * NI * NTH;
-    for (int d = threadIdx.x; d < ndir; d += blockDim.x) {
-        const int it = d % NTH;
-        const DirE w = dirs[d];
-        const float nu = fmaf(sx, w.wx, sz * w.wz);
-        float pos = fmaf(nu, posA, posB);
-        float fl = floorf(pos);
-        float f = pos - fl;
-        int kk = min(max((int)fl, 0), NU);
-        const float4 *pp = pairs + ((size_t)it * NP + kk) * NT * 2;
-        float4 a = pp[0], dd = pp[1];
-        float l0 = fmaf(f, dd.x, a.x), l1 = fmaf(f, dd.y, a.y), l2 = fmaf(f, dd.z, a.z);
-        if (FIRST) {
-            float4 am = pp[2], dm = pp[3];
-            float m0 = fmaf(f, dm.x, am.x), m1 = fmaf(f, dm.y, am.y), m2 = fmaf(f, dm.z, am.z);
-            const float q1 = fmaf(nu, nu, 1.0f);
-            const float y1 = rsqrtf(fmaf(s.m2g, nu, s.g2p1));
-            const float t1 = q1 * (y1 * y1 * y1);
-            l0 = fmaf(l0, q1, m0 * t1);
-            l1 = fmaf(l1, q1, m1 * t1);
-            l2 = fmaf(l2, q1, m2 * t1);
+    for (int c0 = 0; c0 < NTH; c0 += CH) {
+        const int nth = min(CH, NTH - c0);
+        if (c0 > 0) __syncthreads();   // every thread is done with the previous chunk's rows
+        for (int e = threadIdx.x; e < nth * NV; e += blockDim.x) {
+            const int it = c0 + e / NV, kk = e % NV;
+            const double ct = cos((it + 0.5) * (kPI / NI));
+            const double rmu = r * ct;
+            double tz, ty;
+            tex4d_rmu(s, rho2, rmu, rmu * rmu - rho2, tz, ty);
+            float4 v[NT];
+            stage_slice<NT>(s, tabs, tz, ty, txs, kk - 1, v);
+            if (FIRST) {
+                v[0].x *= s.kR; v[0].y *= s.kR; v[0].z *= s.kR;
+                v[NT - 1].x *= s.kM; v[NT - 1].y *= s.kM; v[NT - 1].z *= s.kM;
+            }
+    #pragma unroll
+            for (int n = 0; n < NT; ++n) vtmp[(size_t)e * NT + n] = v[n];
         }
-        a0 = fmaf(l0, w.wzdw, a0);
-        a1 = fmaf(l1, w.wzdw, a1);
-        a2 = fmaf(l2, w.wzdw, a2);
+        __syncthreads();
+        for (int e = threadIdx.x; e < nth * NP * NT; e += blockDim.x) {
+            const int n = e % NT, ek = e / NT;
+            const int it = ek / NP, kk = ek % NP;
+            float4 a = vtmp[((size_t)it * NV + kk) * NT + n];
+            float4 b = vtmp[((size_t)it * NV + kk + 1) * NT + n];
+            pairs[((size_t)(it * NP + kk) * NT + n) * 2] = a;
+            pairs[((size_t)(it * NP + kk) * NT + n) * 2 + 1] =
+                make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.f);
+        }
+        __syncthreads();
+
+        // directions d = phi * NTH + theta of this chunk's theta rows
+        const int ndir = 2 * NI * nth;
+        for (int dl = threadIdx.x; dl < ndir; dl += blockDim.x) {
+            const int it = dl % nth;   // staged row
+            const DirE w = dirs[(dl / nth) * NTH + c0 + it];
+            const float nu = fmaf(sx, w.wx, sz * w.wz);
+            float pos = fmaf(nu, posA, posB);
+            float fl = floorf(pos);
+            float f = pos - fl;
+            int kk = min(max((int)fl, 0), NU);
+            const float4 *pp = pairs + ((size_t)it * NP + kk) * NT * 2;
+            float4 a = pp[0], dd = pp[1];
+            float l0 = fmaf(f, dd.x, a.x), l1 = fmaf(f, dd.y, a.y), l2 = fmaf(f, dd.z, a.z);
+            if (FIRST) {
+                float4 am = pp[2], dm = pp[3];
+                float m0 = fmaf(f, dm.x, am.x), m1 = fmaf(f, dm.y, am.y), m2 = fmaf(f, dm.z, am.z);
+                const float q1 = fmaf(nu, nu, 1.0f);
+                const float y1 = rsqrtf(fmaf(s.m2g, nu, s.g2p1));
+                const float t1 = q1 * (y1 * y1 * y1);
+                l0 = fmaf(l0, q1, m0 * t1);
+                l1 = fmaf(l1, q1, m1 * t1);
+                l2 = fmaf(l2, q1, m2 * t1);
+            }
+            a0 = fmaf(l0, w.wzdw, a0);
+            a1 = fmaf(l1, w.wzdw, a1);
+            a2 = fmaf(l2, w.wzdw, a2);
+        }
     }
     // block reduction
 #pragma unroll
@@ -1004,15 +1041,25 @@ __global__ __launch_bounds__(256) void k_check(const SetDesc *__restrict__ sets,
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-size_t lds_inscatterS(int NS, int RNU, int SW, bool first) {
+size_t lds_inscatterS(int CH, int RNU, int SW, bool first) {
     const int NT = first ? 2 : 1, PW = first ? 3 : 2;
-    return sizeof(float4) * ((size_t)NS * (RNU + 1) * PW + (size_t)NS * (RNU + 2) * NT +
-                             (size_t)(e_pad(SW) + SW + 1) * 2 + (size_t)NS * 2);
+    return sizeof(float4) * ((size_t)CH * (RNU + 1) * PW + (size_t)CH * (RNU + 2) * NT +
+                             (size_t)(e_pad(SW) + SW + 1) * 2 + (size_t)CH * 2);
 }
-size_t lds_irradianceN(int NI, int RNU, bool first) {
+size_t lds_irradianceN(int CH, int RNU, bool first) {
     const int NT = first ? 2 : 1;
-    return sizeof(float4) * ((size_t)(NI / 2) * (RNU + 1) * NT * 2 + (size_t)(NI / 2) * (RNU + 2) * NT) +
+    return sizeof(float4) * ((size_t)CH * (RNU + 1) * NT * 2 + (size_t)CH * (RNU + 2) * NT) +
            sizeof(float) * 3 * 16;
+}
+int chunk_inscatterS(int NS, int RNU, int SW, bool first) {
+    for (int ch = NS; ch >= 1; --ch)
+        if (lds_inscatterS(ch, RNU, SW, first) <= kSphLdsBudget) return ch;
+    return 0;
+}
+int chunk_irradianceN(int NI, int RNU, bool first) {
+    for (int ch = NI / 2; ch >= 1; --ch)
+        if (lds_irradianceN(ch, RNU, first) <= kSphLdsBudget) return ch;
+    return 0;
 }
 
 hipError_t launch_transmittance(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
@@ -1033,36 +1080,41 @@ hipError_t launch_inscatter1(const SetDesc *sets, const uint2 *tiles, int ntiles
     hipLaunchKernelGGL(k_inscatter1, dim3(ntiles), dim3(256), 0, st, sets, tiles, nsamp);
     return hipGetLastError();
 }
-hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS,
+hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS, int CH,
                              const void *dirT, const void *dirW, bool first, size_t lds, int G,
                              hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
-#define ATMO_LJ(F, GG)                                                                          \
-    hipLaunchKernelGGL((k_inscatterS<F, GG>), dim3(ntiles), dim3(256), lds, st, sets, tiles, NS, \
-                       (const DirJ *)dirT, (const float2 *)dirW)
-    if (G == 2) {
-        if (first) ATMO_LJ(true, 2); else ATMO_LJ(false, 2);
+    if (CH < 1 || CH > NS) return hipErrorInvalidValue;
+#define ATMO_LJ(F, GG, C)                                                                          \
+    hipLaunchKernelGGL((k_inscatterS<F, GG, C>), dim3(ntiles), dim3(256), lds, st, sets, tiles, NS, \
+                       CH, (const DirJ *)dirT, (const float2 *)dirW)
+    if (G != 2 && G != 3 && G != 4 && G != 8) return hipErrorInvalidValue;
+    if (CH < NS) {   // chunked staging: one variant (all G give the same bits)
+        if (first) ATMO_LJ(true, 4, true); else ATMO_LJ(false, 4, true);
+    } else if (G == 2) {
+        if (first) ATMO_LJ(true, 2, false); else ATMO_LJ(false, 2, false);
     } else if (G == 3) {
-        if (first) ATMO_LJ(true, 3); else ATMO_LJ(false, 3);
+        if (first) ATMO_LJ(true, 3, false); else ATMO_LJ(false, 3, false);
     } else if (G == 4) {
-        if (first) ATMO_LJ(true, 4); else ATMO_LJ(false, 4);
+        if (first) ATMO_LJ(true, 4, false); else ATMO_LJ(false, 4, false);
     } else if (G == 8) {
-        if (first) ATMO_LJ(true, 8); else ATMO_LJ(false, 8);
+        if (first) ATMO_LJ(true, 8, false); else ATMO_LJ(false, 8, false);
     } else {
         return hipErrorInvalidValue;
     }
 #undef ATMO_LJ
     return hipGetLastError();
 }
-hipError_t launch_irradianceN(const SetDesc *sets, const uint2 *tiles, int ntiles, int NI,
+hipError_t launch_irradianceN(const SetDesc *sets, const uint2 *tiles, int ntiles, int NI, int CH,
                               const void *dirs, bool first, size_t lds, hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
+    if (CH < 1 || CH > NI / 2) return hipErrorInvalidValue;
     if (first)
         hipLaunchKernelGGL(k_irradianceN<true>, dim3(ntiles), dim3(256), lds, st, sets, tiles, NI,
-                           (const DirE *)dirs);
+                           CH, (const DirE *)dirs);
     else
         hipLaunchKernelGGL(k_irradianceN<false>, dim3(ntiles), dim3(256), lds, st, sets, tiles, NI,
-                           (const DirE *)dirs);
+                           CH, (const DirE *)dirs);
     return hipGetLastError();
 }
 size_t lds_inscatterN(int max_row_texels, int sb) {

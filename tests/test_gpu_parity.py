@@ -219,6 +219,25 @@ def _params(v):
     return p
 
 
+@pytest.mark.parametrize("case", ["spherical64", "maximum"])
+def test_direction_grids_staged_in_rounds(eng, case):
+    """Direction grids whose LDS staging exceeds one round: k_inscatterS and
+    k_irradianceN then stage their theta rows in chunks.  spherical64: 64 x 128
+    directions with RES_NU 16 (order 2 in two rounds); maximum: the largest
+    grids and RES_NU the options allow (128 x 256 spherical directions, 256 x
+    512 hemisphere directions, RES_NU 64: up to 13 rounds)."""
+    if case == "spherical64":
+        p = small(res=(4, 16, 4, 16))
+        o = options(spherical_samples=64)
+    else:
+        p = small(res=(2, 8, 2, 64))
+        o = options(spherical_samples=128, irradiance_samples=256)
+    T, E, S = _chain(eng, p, o)
+    oT, oE, oS = O.precompute(p, o)
+    _assert("E", E, oE, TOL)
+    _assert("S", S, oS, TOL)
+
+
 def test_golden_tiny_fixtures(eng):
     g = np.load(os.path.join(GOLDEN, "tiny_earth.npz"))
     p = _params(g["params"])
