@@ -140,13 +140,14 @@ def main():
     for name, order, ms in times:
         by.setdefault(name, []).append(ms)
     pass_ms = {k: sum(v) for k, v in by.items()}
-    # dominant kernel: the dJ pass of orders >= 3 (k_inscatterS<false, G>, one
-    # launch per order; order 2 is its own instantiation with twice the lookups)
+    # dominant kernel: the dJ pass of orders >= 3 (k_inscatterS<false, G, false>,
+    # one launch per order; order 2 is its own instantiation with twice the
+    # lookups; the third argument is the staging-in-rounds variant)
     later = [ms for name, order, ms in times if name == "inscatterS" and order >= 3]
     avg_ms = sum(later) / len(later)
     flop_per_launch = p.n_texels4d() * F.inscatterS_texel(o.spherical_samples, False)
     achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12
-    kname = "k_inscatterS<false,%d>" % eng.tuning().get("inscatterS_texels_per_thread", 4)
+    kname = "k_inscatterS<false,%d,false>" % eng.tuning().get("inscatterS_texels_per_thread", 4)
     traffic = profiled(TRAFFIC_PROFILE, "traffic", kname)
     executed = profiled(COUNTER_PROFILE, "executed_fp32_flop", kname)
     roof = {"bound": "valu", "achieved": round(achieved, 2), "peak": FP32_PEAK_TFLOPS,
@@ -173,18 +174,25 @@ def main():
         roof["hbm_frac"] = round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     whole_tflops = F.total(p, o) / sec / 1e12
 
-    # second kernel: the dSn pass, whose bound is its HBM traffic
+    # second kernel: the dSn pass.  Its HBM traffic is a fraction of peak; what
+    # it moves is on-chip: per view ray (r layer, mu row) and sample, the staged
+    # form blends the 4 dJ rows around the sample (2 layers x 2 mu rows,
+    # RES_NU*RES_MU_S texels of 12 bytes each) into LDS.
     tu = eng.tuning()
     sn_name = "k_inscatterN<%s>" % ("true" if tu.get("inscatterN_staged_samples", 0) else "false")
     sn_ms = [ms for name, order, ms in times if name == "inscatterN"]
     sn_avg = sum(sn_ms) / len(sn_ms)
     sn_traffic = profiled(TRAFFIC_PROFILE, "traffic", sn_name)
-    second = {"kernel": sn_name + " (dSn pass, orders 2-%d)" % o.orders, "bound": "hbm",
-              "avg_launch_ms": round(sn_avg, 4), "launches": len(sn_ms), "traffic": sn_traffic,
-              "unit": "GB/s", "peak": HBM_PEAK_GBS}
+    row_reads = (p.RES_R * p.RES_MU * (o.inscatter_samples + 1) * 4 * p.RES_NU * p.RES_MU_S * 12)
+    second = {"kernel": sn_name + " (dSn pass, orders 2-%d)" % o.orders,
+              "bound": "on-chip: dJ rows gathered per view-ray sample (L2 -> LDS)",
+              "avg_launch_ms": round(sn_avg, 4), "launches": len(sn_ms),
+              "row_gather_bytes": row_reads,
+              "row_gather_tbs": round(row_reads / (sn_avg * 1e-3) / 1e12, 2),
+              "hbm_traffic": sn_traffic, "hbm_peak_gbs": HBM_PEAK_GBS}
     if sn_traffic:
-        second["achieved"] = round(sn_traffic / (sn_avg * 1e-3) / 1e9, 1)
-        second["frac"] = round(sn_traffic / (sn_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        second["hbm_gbs"] = round(sn_traffic / (sn_avg * 1e-3) / 1e9, 1)
+        second["hbm_frac"] = round(sn_traffic / (sn_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
         second["traffic_source"] = TRAFFIC_PROFILE
 
     extra = {"pass_ms": {k: round(v, 3) for k, v in pass_ms.items()},

@@ -170,13 +170,16 @@ ATMO_API atmo_status atmo_engine_pass_times(atmo_engine *e, atmo_pass_time *out,
  * times every variant of the two dominant passes and keeps the fastest.
  * k_inscatterS variants (2 / 3 / 4 / 8 texels per thread, or each set with
  * the G whose workgroup-occupancy cost ceil(RES_MU * ceil(RES_NU / G) / 256) * G
- * is lowest, reported as 0) give bit-identical tables; the two k_inscatterN forms (direct gathers / LDS-staged blended
- * rows) compute the same trilinear filter in a different FP32 order.  Times in
- * ms: inscatterS_ms[i] for 2 / 3 / 4 / 8 / per-set texels per thread (one order-2 +
- * (orders-2) later launches), inscatterN_ms[9f + 3c + x] for form f (direct,
+ * is lowest, reported as 0) give bit-identical tables; the two k_inscatterN
+ * forms (direct gathers / LDS-staged blended rows) compute the same trilinear
+ * filter in a different FP32 order.  Times in ms: inscatterS_ms[i] for
+ * 2 / 3 / 4 / 8 / per-set texels per thread (one order-2 + (orders-2) later
+ * launches), inscatterN_ms[9f + 3c + x] for form f (direct,
  * staged), at most 1 / 2 / 4 mu rows per workgroup (c), tile order x (0 mu
  * rows, 1 XCD-banded mu rows, 2 XCD-banded view lines sorted by impact
- * parameter); -1 = does not fit the device's LDS. */
+ * parameter); -1 = not timed (a k_inscatterN form that does not fit the
+ * device's LDS, or the per-set k_inscatterS choice when every set makes the
+ * same one). */
 typedef struct atmo_tuning {
     int tuned;
     int inscatterS_texels_per_thread;  /* 2, 3, 4, 8; 0: per set */

@@ -624,9 +624,16 @@ atmo_status autotune(Plan &pl, hipStream_t st, Tuning &tu) {
     const uint2 *T = (const uint2 *)pl.dtiles.p;
     const char *dirs = (const char *)pl.ddirs.p;
     HIPCHK(hipStreamSynchronize(st));
-    const int Gs[5] = {2, 3, 4, 8, 0};   // 0: each set with its own G (jg_for_set)
+    // 0: each set with its own G (jg_for_set), a candidate only when the sets'
+    // choices differ (otherwise it is the same launch as one of the others)
+    const int Gs[5] = {2, 3, 4, 8, 0};
+    const bool mixedG = pl.jseg.size() > 1;
     float bestJ = 1e30f;
     for (int gi = 0; gi < 5; ++gi) {
+        if (Gs[gi] == 0 && !mixedG) {
+            tu.msJ[gi] = -1.f;
+            continue;
+        }
         float m1 = 0.f, m2 = 0.f;
         atmo_status rc = time_launch(st, m1, [&]() { return launch_dj(pl, S, T, dirs, true, Gs[gi], st); });
         if (rc != ATMO_OK) return rc;
