@@ -81,9 +81,10 @@ typedef struct atmo_options {
     int texels_per_thread;      /* k_inscatterS variant 2, 3, 4 or 8 (bit-identical results);
                                    0 = per set by workgroup occupancy for the pass-level
                                    entry points, autotuned in atmo_engine_precompute */
-    int inscatterN_form;        /* k_inscatterN form: 1 direct gathers, 2 LDS-staged rows;
-                                   0 = direct for the pass-level entry points, autotuned
-                                   in atmo_engine_precompute */
+    int inscatterN_form;        /* k_inscatterN form: 1 direct gathers, 2 LDS-staged rows
+                                   (2 samples per LDS round), 3 LDS-staged rows (1 sample
+                                   per round); 0 = direct for the pass-level entry points,
+                                   autotuned in atmo_engine_precompute */
     int reserved[8];
 } atmo_options;
 
@@ -174,8 +175,9 @@ ATMO_API atmo_status atmo_engine_pass_times(atmo_engine *e, atmo_pass_time *out,
  * forms (direct gathers / LDS-staged blended rows) compute the same trilinear
  * filter in a different FP32 order.  Times in ms: inscatterS_ms[i] for
  * 2 / 3 / 4 / 8 / per-set texels per thread (one order-2 + (orders-2) later
- * launches), inscatterN_ms[9f + 3c + x] for form f (direct,
- * staged), at most 1 / 2 / 4 mu rows per workgroup (c), tile order x (0 mu
+ * launches), inscatterN_ms[9f + 3c + x] for form f (direct, staged with 1
+ * sample per LDS round, staged with 2), at most 1 / 2 / 4 mu rows per
+ * workgroup (c), tile order x (0 mu
  * rows, 1 XCD-banded mu rows, 2 XCD-banded view lines sorted by impact
  * parameter); -1 = not timed (a k_inscatterN form that does not fit the
  * device's LDS, or the per-set k_inscatterS choice when every set makes the
@@ -186,7 +188,7 @@ typedef struct atmo_tuning {
     int inscatterN_rows_per_workgroup;
     int inscatterN_xcd_order;      /* 0, 1 or 2 (see inscatterN_ms) */
     float inscatterS_ms[5];
-    float inscatterN_ms[18];
+    float inscatterN_ms[27];
     int inscatterN_staged_samples;  /* 0: direct gathers; > 0: staged, samples per LDS round */
 } atmo_tuning;
 ATMO_API atmo_status atmo_engine_tuning(atmo_engine *e, atmo_tuning *out);

@@ -105,8 +105,8 @@ atmo_status validate_opts(const atmo_options &o) {
     if (o.texels_per_thread != 0 && o.texels_per_thread != 2 && o.texels_per_thread != 3 &&
         o.texels_per_thread != 4 && o.texels_per_thread != 8)
         return fail(ATMO_ERR_INVALID_ARGUMENT, "texels_per_thread must be 0, 2, 3, 4 or 8");
-    if (o.inscatterN_form < 0 || o.inscatterN_form > 2)
-        return fail(ATMO_ERR_INVALID_ARGUMENT, "inscatterN_form must be 0, 1 or 2");
+    if (o.inscatterN_form < 0 || o.inscatterN_form > 3)
+        return fail(ATMO_ERR_INVALID_ARGUMENT, "inscatterN_form must be 0, 1, 2 or 3");
     return ATMO_OK;
 }
 
@@ -249,13 +249,11 @@ struct JSeg {   // a run of P_J tiles whose sets share one G
 struct Tuning {
     int jG = 4, snCap = 4, snXcd = 1, snSb = 0;   // jG = 0: per-set choice
     float msJ[5] = {0, 0, 0, 0, 0};  // G = 2, 3, 4, 8, per set (order-2 + order->=3 launches)
-    float msN[18] = {};           // [direct, staged 2][caps 1, 2, 4][plain, XCD, XCD + line order]
+    float msN[27] = {};   // [direct, staged 1, staged 2][caps 1, 2, 4][plain, XCD, XCD + line order]
 };
 
-// static LDS of k_inscatterN (sample staging), on top of its dynamic row blend
-constexpr size_t kSnStaticLds = 16400;
-// samples per LDS round of the staged k_inscatterN form
-constexpr int kSnStagedSamples = 2;
+// staged samples per LDS round of a pinned k_inscatterN form (1 direct: 0)
+int form_sb(int form) { return form == 2 ? 2 : (form == 3 ? 1 : 0); }
 // largest rows-per-workgroup x RNU*RMU_S of a batch (k_inscatterN staging)
 int sn_max_row_texels(const std::vector<atmo_params> &ps, int snCap) {
     int m = 0;
@@ -652,16 +650,16 @@ atmo_status autotune(Plan &pl, hipStream_t st, Tuning &tu) {
     DevBuf ds, dt;
     HIPCHK(ds.ensure(hs.size() * sizeof(SetDesc)));
     HIPCHK(hipMemcpy(ds.p, hs.data(), hs.size() * sizeof(SetDesc), hipMemcpyHostToDevice));
-    const int caps[3] = {1, 2, 4}, sbs[2] = {0, kSnStagedSamples};
+    const int caps[3] = {1, 2, 4}, sbs[3] = {0, 1, 2};   // sb: staged samples per LDS round
     int dev = 0, maxLds = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     float bestN = 1e30f;
-    for (int si = 0; si < 2; ++si)
+    for (int si = 0; si < 3; ++si)
     for (int ci = 0; ci < 3; ++ci)
         for (int xo = 0; xo < 3; ++xo) {
             const size_t lds = lds_inscatterN(sn_max_row_texels(pl.params, caps[ci]), sbs[si]);
-            if (lds + kSnStaticLds > (size_t)maxLds) {  // does not fit this device's LDS
+            if (lds + static_lds_inscatterN() > (size_t)maxLds) {  // does not fit this device's LDS
                 tu.msN[si * 9 + ci * 3 + xo] = -1.f;
                 continue;
             }
@@ -694,7 +692,7 @@ atmo_status apply_tuning(Plan &pl, const Tuning &tu) {
     pl.jG = pl.opts.texels_per_thread ? pl.opts.texels_per_thread : tu.jG;
     pl.snCap = tu.snCap;
     pl.snXcd = tu.snXcd;
-    pl.snSb = pl.opts.inscatterN_form ? (pl.opts.inscatterN_form == 2 ? kSnStagedSamples : 0) : tu.snSb;
+    pl.snSb = pl.opts.inscatterN_form ? form_sb(pl.opts.inscatterN_form) : tu.snSb;
     pl.tuned = true;
     if (retile) {
         std::vector<uint2> tiles;
@@ -712,7 +710,7 @@ atmo_status build_plan(atmo_engine *e, const atmo_params *sets, int nsets, const
     pl->params.assign(sets, sets + nsets);
     pl->opts = o;
     pl->jG = o.texels_per_thread;   // 0: each set with its own G until the tuner has run
-    pl->snSb = o.inscatterN_form == 2 ? kSnStagedSamples : 0;
+    pl->snSb = form_sb(o.inscatterN_form);
     // work arena
     size_t total = 0;
     std::vector<size_t> base(nsets);
@@ -864,7 +862,7 @@ atmo_status atmo_engine_tuning(atmo_engine *e, atmo_tuning *out) {
     out->inscatterN_rows_per_workgroup = e->last_tuning.snCap;
     out->inscatterN_xcd_order = e->last_tuning.snXcd;
     for (int i = 0; i < 5; ++i) out->inscatterS_ms[i] = e->last_tuning.msJ[i];
-    for (int i = 0; i < 18; ++i) out->inscatterN_ms[i] = e->last_tuning.msN[i];
+    for (int i = 0; i < 27; ++i) out->inscatterN_ms[i] = e->last_tuning.msN[i];
     out->inscatterN_staged_samples = e->last_tuning.snSb;
     return ATMO_OK;
 }
@@ -1070,7 +1068,7 @@ static atmo_status pass_run(atmo_engine *e, const atmo_params *p, const atmo_opt
         break;
     }
     case P_SN: {
-        const int sb = o.inscatterN_form == 2 ? kSnStagedSamples : 0;
+        const int sb = form_sb(o.inscatterN_form);
         const size_t lds = lds_inscatterN(sn_max_row_texels(std::vector<atmo_params>(1, *p), 4), sb);
         HIPCHK(launch_inscatterN(S, T + off[P_SN], cnt[P_SN], o.inscatter_samples, 4, sb, lds, hs));
         break;

@@ -42,6 +42,7 @@ hipError_t launch_irradianceN(const SetDesc *sets, const uint2 *tiles, int ntile
 // LDS-staged blended rows, sb samples per round, lds = lds_inscatterN(max over
 // the batch of rows-per-workgroup x RNU*RMU_S, sb) bytes of dynamic LDS.
 size_t lds_inscatterN(int max_row_texels, int sb);
+size_t static_lds_inscatterN();   // bytes of k_inscatterN's static LDS
 hipError_t launch_inscatterN(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
                              int rpw_cap, int sb, size_t lds, hipStream_t st);
 hipError_t launch_check(const SetDesc *sets, const uint2 *tiles, int ntiles, unsigned int *flags,

@@ -767,8 +767,8 @@ template <bool STAGED>
 __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ sets,
                                                     const uint2 *__restrict__ tiles,
                                                     int nsamp, int rpw_cap, int sb) {
-    __shared__ SnEntry ent[kNRows][kNChunk];
-    __shared__ float rowdx[kNRows];
+    __shared__ SnEntry ent[kNRows][kNChunk];   // exactly 16 KiB: two workgroups of 64 KiB
+                                               // dynamic staging fit one CU's 160 KiB
     extern __shared__ float4 blend[];   // [sb][nrows * W4] (STAGED)
     const uint2 tl = tiles[blockIdx.x];
     const SetDesc &s = sets[tl.x];
@@ -787,12 +787,12 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
     const int nmine = (per + (int)blockDim.x - 1) / (int)blockDim.x;
     constexpr int MAXT = 4;   // W4 <= 1024 (validated on the host)
     float acc[MAXT][3];
-    float nuf[MAXT], muSr[MAXT], fk[MAXT];
+    float nuf[MAXT], muSr[MAXT], fk[MAXT], dxj[MAXT];
     int k0[MAXT], rowi[MAXT];
 #pragma unroll
     for (int j = 0; j < MAXT; ++j) {
         acc[j][0] = acc[j][1] = acc[j][2] = 0.f;
-        nuf[j] = muSr[j] = fk[j] = 0.f;
+        nuf[j] = muSr[j] = fk[j] = dxj[j] = 0.f;
         k0[j] = rowi[j] = 0;
         const int e = threadIdx.x + j * (int)blockDim.x;
         if (j < nmine && e < per) {
@@ -809,6 +809,8 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
             nuf[j] = (float)nu;
             muSr[j] = (float)(muS * r);
             rowi[j] = row;
+            // the row's trapezoid step, as the sample staging computes it (mu depends on the row only)
+            dxj[j] = (float)(limit_d(s, r, mu) / nsamp);
         }
     }
 
@@ -830,7 +832,6 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
             const double rmu = r * mu;
             const double delta = rmu * rmu - rho2;            // constant along the ray
             const double dx = limit_d(s, r, mu) / nsamp;
-            if (i == 0) rowdx[row] = (float)dx;
             const double ti = i * dx;
             const double q = rho2 + ti * (ti + 2.0 * rmu);   // ri^2 - Rg^2
             const double rmui = rmu + ti;                     // ri * mui
@@ -995,7 +996,7 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
         if (e >= per) break;
         const int row = e / s.W4, x = e % s.W4;
         const size_t t = ((size_t)l * s.RMU + yb + row) * s.W4 + x;
-        const float dx = rowdx[row];
+        const float dx = dxj[j];
         const float4 o = q16(make_float4(acc[j][0] * dx, acc[j][1] * dx, acc[j][2] * dx, 0.f), s.hs);
         s.dSR4[t] = o;
         if (s.S4) {
@@ -1117,6 +1118,7 @@ hipError_t launch_irradianceN(const SetDesc *sets, const uint2 *tiles, int ntile
                            CH, (const DirE *)dirs);
     return hipGetLastError();
 }
+size_t static_lds_inscatterN() { return sizeof(SnEntry) * kNRows * kNChunk; }
 size_t lds_inscatterN(int max_row_texels, int sb) {
     return sb > 0 ? sizeof(float4) * (size_t)sb * (size_t)max_row_texels : 0;
 }

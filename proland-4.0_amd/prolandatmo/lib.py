@@ -33,7 +33,7 @@ PASS_NAMES = ("transmittance", "irradiance1", "inscatter1", "inscatterS", "irrad
 class AtmoTuning(ctypes.Structure):
     _fields_ = [("tuned", ctypes.c_int), ("inscatterS_texels_per_thread", ctypes.c_int),
                 ("inscatterN_rows_per_workgroup", ctypes.c_int), ("inscatterN_xcd_order", ctypes.c_int),
-                ("inscatterS_ms", ctypes.c_float * 5), ("inscatterN_ms", ctypes.c_float * 18),
+                ("inscatterS_ms", ctypes.c_float * 5), ("inscatterN_ms", ctypes.c_float * 27),
                 ("inscatterN_staged_samples", ctypes.c_int)]
 
     def as_dict(self):
