@@ -9,7 +9,7 @@ torch = pytest.importorskip("torch")
 from oracle import mlp_oracle as M  # noqa: E402
 from prolandatmo import lib as L  # noqa: E402
 from prolandatmo import surrogate as SG  # noqa: E402
-from prolandatmo.params import SEED_BATCH, draw_sets, options  # noqa: E402
+from prolandatmo.params import FLAG_HALF_STORAGE, SEED_BATCH, draw_sets, options  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -116,6 +116,16 @@ def test_judge_rejects_and_falls_back_to_plain(sur):
         assert np.array_equal(T.double().cpu().numpy(), rT)
         assert np.array_equal(E.double().cpu().numpy(), rE)
         assert np.array_equal(S.double().cpu().numpy(), rS)
+
+
+def test_surrogate_rejects_half_storage(sur):
+    """The surrogate predicts FP32 tables: FP16 storage is refused, not mixed in."""
+    sets = small_sets(1, seed=SEED_BATCH + 3)
+    o = options(flags=FLAG_HALF_STORAGE)
+    with pytest.raises(L.AtmoError, match="FP16|HALF"):
+        sur.precompute(sets, o)
+    with pytest.raises(L.AtmoError, match="FP16|HALF"):
+        sur.build_dataset(sets, o, samples_per_set=64)
 
 
 def test_judge_accepts_and_keeps_exact_parts(sur):
