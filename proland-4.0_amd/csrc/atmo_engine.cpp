@@ -871,8 +871,13 @@ atmo_status atmo_engine_workspace_size(const atmo_params *sets, int nsets, const
                                        size_t *bytes) {
     if (!sets || nsets <= 0 || !bytes) return fail(ATMO_ERR_INVALID_ARGUMENT, "bad arguments");
     atmo_options o = opts ? *opts : default_options();
+    atmo_status st = validate_opts(o);
+    if (st != ATMO_OK) return st;
     size_t t = 0;
-    for (int i = 0; i < nsets; ++i) t += work_bytes(sets[i], o);
+    for (int i = 0; i < nsets; ++i) {
+        if ((st = validate(sets[i], i)) != ATMO_OK) return st;
+        t += work_bytes(sets[i], o);
+    }
     *bytes = t;
     return ATMO_OK;
 }

@@ -102,6 +102,21 @@ def test_workspace_size_counts_work_tables():
     assert b1 < b
 
 
+@pytest.mark.parametrize("bad", [dict(texels_per_thread=5), dict(texels_per_thread=-1),
+                                 dict(inscatterN_form=4), dict(orders=0), dict(spherical_samples=129),
+                                 dict(irradiance_samples=7)])
+def test_invalid_options_are_rejected_before_any_device_call(bad):
+    with pytest.raises(L.AtmoError) as ei:
+        L.workspace_size(earth(), options(**bad))
+    assert ei.value.status == 1
+
+
+@pytest.mark.parametrize("good", [dict(texels_per_thread=3), dict(inscatterN_form=3),
+                                  dict(flags=16), dict(spherical_samples=128, irradiance_samples=256)])
+def test_option_ranges_accepted(good):
+    assert L.workspace_size(earth(), options(**good)) > 0
+
+
 def test_engine_create_reports_missing_device():
     import torch
     if torch.cuda.is_available():
