@@ -188,6 +188,37 @@ def test_half_storage_is_part_of_the_plan(eng):
     assert not np.array_equal(a[2], b[2])
 
 
+def _extreme_sets():
+    """Corners of the batch distribution: thin and thick atmospheres, dense
+    grey haze with a sharp Mie peak, weak Rayleigh, a small planet."""
+    out = []
+    for (Rg, dR, HR, sR, HM, bM, g) in ((3000.0, 40.0, 5.0, 3.0, 0.6, 3e-2, 0.9),
+                                        (7000.0, 120.0, 12.0, 0.3, 2.4, 1e-3, 0.6),
+                                        (6360.0, 60.0, 8.0, 1.0, 1.2, 3e-2, 0.9)):
+        q = small()
+        q.Rg, q.Rt, q.RL = Rg, Rg + dR, Rg + dR + 1.0
+        q.HR, q.HM, q.mieG = HR, HM, g
+        q.betaR.x, q.betaR.y, q.betaR.z = 5.8e-3 * sR, 1.35e-2 * sR, 3.31e-2 * sR
+        q.betaMSca.x = q.betaMSca.y = q.betaMSca.z = bM
+        q.betaMEx.x = q.betaMEx.y = q.betaMEx.z = bM / 0.9
+        out.append(q)
+    return out
+
+
+def test_batch_of_seeded_and_extreme_atmospheres_matches_oracle(eng):
+    """Eight sets of the batch distribution (seeded, a stream of their own) and
+    three corners of it, in one batch at 8x32x8x4: every table of every set
+    within the 1e-3 bar of the oracle."""
+    sets = seeded(8, (8, 32, 8, 4), seed=SEED_BATCH + 17) + _extreme_sets()
+    outs = eng.precompute(sets, options())
+    torch.cuda.synchronize()
+    for i, (p, (T, E, S)) in enumerate(zip(sets, outs)):
+        oT, oE, oS = O.precompute(p, options())
+        _assert("T[%d]" % i, T.double().cpu().numpy(), oT, TOL)
+        _assert("E[%d]" % i, E.double().cpu().numpy(), oE, TOL)
+        _assert("S[%d]" % i, S.double().cpu().numpy(), oS, TOL)
+
+
 def test_chain_orders_1_and_6(eng):
     p = small()
     for n in (1, 6):
