@@ -99,6 +99,13 @@ typedef struct atmo_options {
                                        RGB16F / RGBA16F textures and blending do
                                        (PreprocessAtmo.cpp:130-143, :336, :350); the
                                        default keeps FP32 tables */
+#define ATMO_FLAG_LINEAR_TRANSMITTANCE 32  /* the linear transmittance parameterisation: the
+                                              reference built without TRANSMITTANCE_NON_LINEAR
+                                              (common.glsl:105, :113-136) */
+#define ATMO_FLAG_LINEAR_INSCATTER     64  /* the linear inscatter parameterisation: the
+                                              reference built without INSCATTER_NON_LINEAR
+                                              (common.glsl:106, :149-201); the table consumer
+                                              (atmo_consumer.h) reads the non-linear tables */
 
 /* Reference defaults (PreprocessAtmo.cpp:90-98; note betaMEx = 4.44e-3). */
 ATMO_API void atmo_params_default(atmo_params *p);

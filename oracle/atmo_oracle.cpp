@@ -51,6 +51,7 @@ struct P {
     double bR[3], bMS[3], bME[3];
     int TW, TH, SW, SH, RR, RMU, RMUS, RNU;
     int orders, tS, iS, irrS, sphS;
+    bool linT, lin4;   /* linear parameterisations (common.glsl:105-106 undefined) */
 };
 
 P mk(const atmo_params *a, const atmo_options *o) {
@@ -66,6 +67,8 @@ P mk(const atmo_params *a, const atmo_options *o) {
     p.RR = a->RES_R; p.RMU = a->RES_MU; p.RMUS = a->RES_MU_S; p.RNU = a->RES_NU;
     /* common.glsl:94-97 and PreprocessAtmo.cpp:351 */
     p.orders = 4; p.tS = 500; p.iS = 50; p.irrS = 32; p.sphS = 16;
+    p.linT = o && (o->flags & ATMO_FLAG_LINEAR_TRANSMITTANCE);
+    p.lin4 = o && (o->flags & ATMO_FLAG_LINEAR_INSCATTER);
     if (o) {
         p.orders = o->orders; p.tS = o->transmittance_samples; p.iS = o->inscatter_samples;
         p.irrS = o->irradiance_samples; p.sphS = o->spherical_samples;
@@ -159,8 +162,14 @@ double phaseM(const P &p, double mu) {
            (1.0 + mu * mu) / (2.0 + g * g);
 }
 
-/* common.glsl:114-124 getTransmittanceUV (TRANSMITTANCE_NON_LINEAR), D3 */
+/* common.glsl:114-124 getTransmittanceUV, D3 (TRANSMITTANCE_NON_LINEAR; the
+ * #else branch :119-120 with linT) */
 void transmittanceUV(const P &p, double r, double mu, double &u, double &v) {
+    if (p.linT) {
+        u = (mu + 0.15) / (1.0 + 0.15);
+        v = (r - p.Rg) / (p.Rt - p.Rg);
+        return;
+    }
     double uR = std::sqrt(std::max(r - p.Rg, 0.0) / (p.Rt - p.Rg));
     double uMu = std::atan((mu + 0.15) / (1.0 + 0.15) * std::tan(1.5)) / 1.5;
     u = uMu;
@@ -171,6 +180,11 @@ void transmittanceUV(const P &p, double r, double mu, double &u, double &v) {
 void transmittanceRMu(const P &p, int i, int j, double &r, double &mu) {
     double rr = (j + 0.5) / p.TH;
     double m = (i + 0.5) / p.TW;
+    if (p.linT) {   /* :132-133 */
+        r = p.Rg + rr * (p.Rt - p.Rg);
+        mu = -0.15 + m * (1.0 + 0.15);
+        return;
+    }
     r = p.Rg + (rr * rr) * (p.Rt - p.Rg);
     mu = -0.15 + std::tan(1.5 * m) / std::tan(1.5) * (1.0 + 0.15);
 }
@@ -213,7 +227,12 @@ void E_lookup(const P &p, const double *E, double r, double muS, double out[3]) 
 /* common.glsl:149-173 texture4D (INSCATTER_NON_LINEAR): the 3D-texture
  * coordinates of the two nu slices and the nu lerp weight.
  * out = {uR, uMu, uMuS, nu-lerp position}. D1, D2. */
+void tex4D_coords_linear(const P &p, double r, double mu, double muS, double nu, double out[4]);
 void tex4D_coords(const P &p, double r, double mu, double muS, double nu, double out[4]) {
+    if (p.lin4) {
+        tex4D_coords_linear(p, r, mu, muS, nu, out);
+        return;
+    }
     double H = std::sqrt(p.Rt * p.Rt - p.Rg * p.Rg);
     double rho = std::sqrt(std::max(r * r - p.Rg * p.Rg, 0.0));
     double rmu = r * mu;
@@ -237,10 +256,24 @@ void tex4D_coords(const P &p, double r, double mu, double muS, double nu, double
     out[3] = lerp;
 }
 
+/* common.glsl:162-166, the #else branch (no INSCATTER_NON_LINEAR) */
+void tex4D_coords_linear(const P &p, double r, double mu, double muS, double nu, double out[4]) {
+    double H = std::sqrt(p.Rt * p.Rt - p.Rg * p.Rg);
+    double rho = std::sqrt(std::max(r * r - p.Rg * p.Rg, 0.0));
+    out[0] = 0.5 / p.RR + rho / H * (1.0 - 1.0 / p.RR);
+    out[1] = 0.5 / p.RMU + (mu + 1.0) / 2.0 * (1.0 - 1.0 / p.RMU);
+    out[2] = 0.5 / p.RMUS + std::max(muS + 0.2, 0.0) / 1.2 * (1.0 - 1.0 / p.RMUS);
+    out[3] = (nu + 1.0) / 2.0 * (p.RNU - 1.0);
+}
+
 /* The literal GLSL form of the same coordinates (common.glsl:155-160), kept
  * only so the tests can show D2 changes nothing away from rho = 0. */
 void tex4D_coords_literal(const P &p, double r, double mu, double muS, double nu,
                           double out[4]) {
+    if (p.lin4) {
+        tex4D_coords_linear(p, r, mu, muS, nu, out);
+        return;
+    }
     double H = std::sqrt(p.Rt * p.Rt - p.Rg * p.Rg);
     double rho = std::sqrt(r * r - p.Rg * p.Rg);
     double rmu = r * mu;
@@ -282,6 +315,13 @@ void muMuSNu(const P &p, double r, const double *dhdH, int xi, int yi, double &m
              double &nu) {
     double x = xi;  /* gl_FragCoord.x - 0.5 */
     double y = yi;
+    if (p.lin4) {   /* :194-199 */
+        mu = -1.0 + 2.0 * y / (p.RMU - 1.0);
+        muS = std::fmod(x, (double)p.RMUS) / (p.RMUS - 1.0);
+        muS = -0.2 + muS * 1.2;
+        nu = -1.0 + std::floor(x / p.RMUS) / (p.RNU - 1.0) * 2.0;
+        return;
+    }
     if (y < p.RMU / 2.0) {
         double d = 1.0 - y / (p.RMU / 2.0 - 1.0);
         d = std::min(std::max(dhdH[2], d * dhdH[3]), dhdH[3] * 0.999);
@@ -342,26 +382,28 @@ void oracle_layer(const atmo_params *a, int l, double *r, double dhdH[4]) {
 double oracle_limit(const atmo_params *a, double r, double mu) { return limit(mk(a, nullptr), r, mu); }
 double oracle_phaseR(double mu) { return phaseR(mu); }
 double oracle_phaseM(const atmo_params *a, double mu) { return phaseM(mk(a, nullptr), mu); }
-void oracle_transmittance_uv(const atmo_params *a, double r, double mu, double uv[2]) {
-    transmittanceUV(mk(a, nullptr), r, mu, uv[0], uv[1]);
+void oracle_transmittance_uv(const atmo_params *a, const atmo_options *o, double r, double mu,
+                             double uv[2]) {
+    transmittanceUV(mk(a, o), r, mu, uv[0], uv[1]);
 }
-void oracle_transmittance_rmu(const atmo_params *a, int i, int j, double rmu[2]) {
-    transmittanceRMu(mk(a, nullptr), i, j, rmu[0], rmu[1]);
+void oracle_transmittance_rmu(const atmo_params *a, const atmo_options *o, int i, int j,
+                              double rmu[2]) {
+    transmittanceRMu(mk(a, o), i, j, rmu[0], rmu[1]);
 }
 void oracle_irradiance_rmus(const atmo_params *a, int i, int j, double rmus[2]) {
     irradianceRMuS(mk(a, nullptr), i, j, rmus[0], rmus[1]);
 }
-void oracle_mumusnu(const atmo_params *a, double r, const double dhdH[4], int x, int y,
-                    double out[3]) {
-    muMuSNu(mk(a, nullptr), r, dhdH, x, y, out[0], out[1], out[2]);
+void oracle_mumusnu(const atmo_params *a, const atmo_options *o, double r, const double dhdH[4],
+                    int x, int y, double out[3]) {
+    muMuSNu(mk(a, o), r, dhdH, x, y, out[0], out[1], out[2]);
 }
-void oracle_texture4D_coords(const atmo_params *a, double r, double mu, double muS, double nu,
-                             double out[4]) {
-    tex4D_coords(mk(a, nullptr), r, mu, muS, nu, out);
+void oracle_texture4D_coords(const atmo_params *a, const atmo_options *o, double r, double mu,
+                             double muS, double nu, double out[4]) {
+    tex4D_coords(mk(a, o), r, mu, muS, nu, out);
 }
-void oracle_texture4D_coords_literal(const atmo_params *a, double r, double mu, double muS,
-                                     double nu, double out[4]) {
-    tex4D_coords_literal(mk(a, nullptr), r, mu, muS, nu, out);
+void oracle_texture4D_coords_literal(const atmo_params *a, const atmo_options *o, double r,
+                                     double mu, double muS, double nu, double out[4]) {
+    tex4D_coords_literal(mk(a, o), r, mu, muS, nu, out);
 }
 void oracle_sample2D(const double *tex, int W, int H, int C, double s, double t, double *out) {
     sample2(tex, W, H, C, s, t, out);
@@ -370,9 +412,9 @@ void oracle_sample3D(const double *tex, int W, int H, int D, int C, double s, do
                      double *out) {
     sample3(tex, W, H, D, C, s, t, u, out);
 }
-void oracle_texture4D(const atmo_params *a, const double *tab, int C, double r, double mu,
-                      double muS, double nu, double *out) {
-    tex4D(mk(a, nullptr), tab, C, r, mu, muS, nu, out);
+void oracle_texture4D(const atmo_params *a, const atmo_options *o, const double *tab, int C,
+                      double r, double mu, double muS, double nu, double *out) {
+    tex4D(mk(a, o), tab, C, r, mu, muS, nu, out);
 }
 void oracle_transmittance_lookup(const atmo_params *a, const double *T, double r, double mu,
                                  double out[3]) {

@@ -13,6 +13,9 @@
  * in FP64; with ATMO_FLAG_HALF_STORAGE, oracle_precompute rounds every stored
  * table to FP16 as the reference's RGB16F/RGBA16F textures do.
  *
+ * The coordinate primitives take the options only for their parameterisation
+ * flags (NULL: the reference's non-linear parameterisations).
+ *
  * Table layouts (doubles, rows as in the reference's .raw files):
  *   T   [TRANSMITTANCE_H][TRANSMITTANCE_W][3]
  *   E   [SKY_H][SKY_W][3]          (also dE)
@@ -35,21 +38,24 @@ ORACLE_API void oracle_layer(const atmo_params *p, int layer, double *r, double 
 ORACLE_API double oracle_limit(const atmo_params *p, double r, double mu);
 ORACLE_API double oracle_phaseR(double mu);
 ORACLE_API double oracle_phaseM(const atmo_params *p, double mu);
-ORACLE_API void oracle_transmittance_uv(const atmo_params *p, double r, double mu, double uv[2]);
-ORACLE_API void oracle_transmittance_rmu(const atmo_params *p, int i, int j, double rmu[2]);
+ORACLE_API void oracle_transmittance_uv(const atmo_params *p, const atmo_options *o, double r,
+                                        double mu, double uv[2]);
+ORACLE_API void oracle_transmittance_rmu(const atmo_params *p, const atmo_options *o, int i, int j,
+                                         double rmu[2]);
 ORACLE_API void oracle_irradiance_rmus(const atmo_params *p, int i, int j, double rmus[2]);
-ORACLE_API void oracle_mumusnu(const atmo_params *p, double r, const double dhdH[4], int x, int y,
-                               double out[3]);
-ORACLE_API void oracle_texture4D_coords(const atmo_params *p, double r, double mu, double muS,
-                                        double nu, double out[4]);
-ORACLE_API void oracle_texture4D_coords_literal(const atmo_params *p, double r, double mu,
-                                                double muS, double nu, double out[4]);
+ORACLE_API void oracle_mumusnu(const atmo_params *p, const atmo_options *o, double r,
+                               const double dhdH[4], int x, int y, double out[3]);
+ORACLE_API void oracle_texture4D_coords(const atmo_params *p, const atmo_options *o, double r,
+                                        double mu, double muS, double nu, double out[4]);
+ORACLE_API void oracle_texture4D_coords_literal(const atmo_params *p, const atmo_options *o,
+                                                double r, double mu, double muS, double nu,
+                                                double out[4]);
 ORACLE_API void oracle_sample2D(const double *tex, int W, int H, int C, double s, double t,
                                 double *out);
 ORACLE_API void oracle_sample3D(const double *tex, int W, int H, int D, int C, double s, double t,
                                 double u, double *out);
-ORACLE_API void oracle_texture4D(const atmo_params *p, const double *tab, int C, double r,
-                                 double mu, double muS, double nu, double *out);
+ORACLE_API void oracle_texture4D(const atmo_params *p, const atmo_options *o, const double *tab,
+                                 int C, double r, double mu, double muS, double nu, double *out);
 ORACLE_API void oracle_transmittance_lookup(const atmo_params *p, const double *T, double r,
                                             double mu, double out[3]);
 ORACLE_API void oracle_transmittance_segment(const atmo_params *p, const double *T, double r,

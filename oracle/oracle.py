@@ -37,17 +37,17 @@ def lib():
             "oracle_limit": (ctypes.c_double, [_pp, ctypes.c_double, ctypes.c_double]),
             "oracle_phaseR": (ctypes.c_double, [ctypes.c_double]),
             "oracle_phaseM": (ctypes.c_double, [_pp, ctypes.c_double]),
-            "oracle_transmittance_uv": (None, [_pp, ctypes.c_double, ctypes.c_double, _dp]),
-            "oracle_transmittance_rmu": (None, [_pp, ctypes.c_int, ctypes.c_int, _dp]),
+            "oracle_transmittance_uv": (None, [_pp, _po, ctypes.c_double, ctypes.c_double, _dp]),
+            "oracle_transmittance_rmu": (None, [_pp, _po, ctypes.c_int, ctypes.c_int, _dp]),
             "oracle_irradiance_rmus": (None, [_pp, ctypes.c_int, ctypes.c_int, _dp]),
-            "oracle_mumusnu": (None, [_pp, ctypes.c_double, _dp, ctypes.c_int, ctypes.c_int, _dp]),
-            "oracle_texture4D_coords": (None, [_pp] + [ctypes.c_double] * 4 + [_dp]),
-            "oracle_texture4D_coords_literal": (None, [_pp] + [ctypes.c_double] * 4 + [_dp]),
+            "oracle_mumusnu": (None, [_pp, _po, ctypes.c_double, _dp, ctypes.c_int, ctypes.c_int, _dp]),
+            "oracle_texture4D_coords": (None, [_pp, _po] + [ctypes.c_double] * 4 + [_dp]),
+            "oracle_texture4D_coords_literal": (None, [_pp, _po] + [ctypes.c_double] * 4 + [_dp]),
             "oracle_sample2D": (None, [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_double, ctypes.c_double, _dp]),
             "oracle_sample3D": (None, [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]),
-            "oracle_texture4D": (None, [_pp, _dp, ctypes.c_int] + [ctypes.c_double] * 4 + [_dp]),
+            "oracle_texture4D": (None, [_pp, _po, _dp, ctypes.c_int] + [ctypes.c_double] * 4 + [_dp]),
             "oracle_transmittance_lookup": (None, [_pp, _dp, ctypes.c_double, ctypes.c_double, _dp]),
             "oracle_transmittance_segment": (None, [_pp, _dp, ctypes.c_double, ctypes.c_double,
                                                     ctypes.c_double, _dp]),
@@ -221,16 +221,20 @@ def phaseM(p, mu):
     return lib().oracle_phaseM(p, mu)
 
 
-def transmittance_uv(p, r, mu):
-    o = np.zeros(2)
-    lib().oracle_transmittance_uv(p, r, mu, _d(o))
-    return o
+def _opt(o):
+    return ctypes.byref(o) if o is not None else None
 
 
-def transmittance_rmu(p, i, j):
-    o = np.zeros(2)
-    lib().oracle_transmittance_rmu(p, i, j, _d(o))
-    return o
+def transmittance_uv(p, r, mu, o=None):
+    out = np.zeros(2)
+    lib().oracle_transmittance_uv(p, _opt(o), r, mu, _d(out))
+    return out
+
+
+def transmittance_rmu(p, i, j, o=None):
+    out = np.zeros(2)
+    lib().oracle_transmittance_rmu(p, _opt(o), i, j, _d(out))
+    return out
 
 
 def irradiance_rmus(p, i, j):
@@ -239,17 +243,18 @@ def irradiance_rmus(p, i, j):
     return o
 
 
-def mumusnu(p, r, dhdH, x, y):
-    o = np.zeros(3)
-    lib().oracle_mumusnu(p, r, _d(np.ascontiguousarray(dhdH, dtype=np.float64)), x, y, _d(o))
-    return o
+def mumusnu(p, r, dhdH, x, y, o=None):
+    out = np.zeros(3)
+    lib().oracle_mumusnu(p, _opt(o), r, _d(np.ascontiguousarray(dhdH, dtype=np.float64)), x, y,
+                         _d(out))
+    return out
 
 
-def texture4D_coords(p, r, mu, muS, nu, literal=False):
-    o = np.zeros(4)
+def texture4D_coords(p, r, mu, muS, nu, literal=False, o=None):
+    out = np.zeros(4)
     f = lib().oracle_texture4D_coords_literal if literal else lib().oracle_texture4D_coords
-    f(p, r, mu, muS, nu, _d(o))
-    return o
+    f(p, _opt(o), r, mu, muS, nu, _d(out))
+    return out
 
 
 def sample2D(tex, s, t):
@@ -268,12 +273,12 @@ def sample3D(tex, s, t, u):
     return o
 
 
-def texture4D(p, tab, r, mu, muS, nu):
+def texture4D(p, tab, r, mu, muS, nu, o=None):
     tab = np.ascontiguousarray(tab, dtype=np.float64)
     C = tab.shape[-1]
-    o = np.zeros(C)
-    lib().oracle_texture4D(p, _d(tab), C, r, mu, muS, nu, _d(o))
-    return o
+    out = np.zeros(C)
+    lib().oracle_texture4D(p, _opt(o), _d(tab), C, r, mu, muS, nu, _d(out))
+    return out
 
 
 def transmittance_lookup(p, T, r, mu):

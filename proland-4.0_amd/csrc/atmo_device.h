@@ -50,6 +50,7 @@ struct SetDesc {
     float4 *T4, *dE4, *E4, *dSR4, *dSM4, *dJ4, *S4;
     int dJb;                        // bytes per dJ texel: 16 (float4) or 12 (packed rgb)
     int hs;                         // 1: table values rounded to FP16 on store (q16)
+    int linT, lin4;                 // 1: the linear transmittance / inscatter parameterisation
     float *outT, *outE;
     const LayerGeom *layers;
 };
@@ -129,12 +130,23 @@ __device__ __forceinline__ float3 bilinear(const float4 *__restrict__ tab, int W
     return lerp3(lerp3(v00, v01, a), lerp3(v10, v11, a), b);
 }
 
-// common.glsl:114-124 + 223-226: T(r, mu) with the altitude h = r - Rg given
-// (computed without cancellation by the caller).
+// getTransmittanceUV (common.glsl:114-124): uR from the altitude h = r - Rg
+// (computed without cancellation by the caller; below the ground it clamps to
+// row 0 in both parameterisations), and uMu.
+__device__ __forceinline__ float T_ur(const SetDesc &s, float h) {
+    return s.linT ? fmaxf(h, 0.0f) * s.f_inv_dRt : sqrtf(fmaxf(h, 0.0f) * s.f_inv_dRt);
+}
+template <bool LIN>   // the parameterisation fixed at compile time (inner loops)
+__device__ __forceinline__ float T_umu_t(const SetDesc &s, float mu) {
+    return LIN ? (mu + 0.15f) * (1.0f / 1.15f) : atanf((mu + 0.15f) * s.f_tMu) * (1.0f / 1.5f);
+}
+__device__ __forceinline__ float T_umu(const SetDesc &s, float mu) {
+    return s.linT ? T_umu_t<true>(s, mu) : T_umu_t<false>(s, mu);
+}
+
+// common.glsl:223-226: T(r, mu), bilinear.
 __device__ __forceinline__ float3 T_lookup(const SetDesc &s, float h, float mu) {
-    float uR = sqrtf(fmaxf(h, 0.0f) * s.f_inv_dRt);
-    float uMu = atanf((mu + 0.15f) * s.f_tMu) * (1.0f / 1.5f);
-    return bilinear(s.T4, s.TW, s.TH, uMu * s.TW - 0.5f, uR * s.TH - 0.5f);
+    return bilinear(s.T4, s.TW, s.TH, T_umu(s, mu) * s.TW - 0.5f, T_ur(s, h) * s.TH - 0.5f);
 }
 
 // Phase functions, common.glsl:290-297.
@@ -160,10 +172,17 @@ __device__ __forceinline__ double limit_d(const SetDesc &s, double r, double mu)
 }
 
 // common.glsl:175-201 getMuMuSNu for texel (x, y) of a layer, FP64.
+__device__ __forceinline__ double texel_mus(const SetDesc &s, int xm);
 __device__ __forceinline__ void mu_mus_nu(const SetDesc &s, const LayerGeom &L, int x, int y,
                                           double &mu, double &muS, double &nu) {
     double r = L.r;
     double half = s.RMU * 0.5;
+    if (s.lin4) {   // linear parameterisation (common.glsl:194-199)
+        mu = -1.0 + 2.0 * y / (s.RMU - 1.0);
+        muS = texel_mus(s, x % s.RMUS);
+        nu = -1.0 + (double)(x / s.RMUS) / (s.RNU - 1.0) * 2.0;
+        return;
+    }
     if (y < half) {
         double d = 1.0 - y / (half - 1.0);
         d = fmin(fmax(L.dhdH[2], d * L.dhdH[3]), L.dhdH[3] * 0.999);
@@ -184,6 +203,7 @@ __device__ __forceinline__ void mu_mus_nu(const SetDesc &s, const LayerGeom &L, 
 // muS of a texel column (depends on x only).
 __device__ __forceinline__ double texel_mus(const SetDesc &s, int xm) {
     double m = xm / (s.RMUS - 1.0);
+    if (s.lin4) return -0.2 + m * 1.2;
     return tan((2.0 * m - 1.0 + 0.26) * 1.1) / tan(1.26 * 1.1);
 }
 
@@ -191,9 +211,14 @@ __device__ __forceinline__ double texel_mus(const SetDesc &s, int xm) {
 // tz (r axis), ty (mu axis), txs (mu_s axis inside a nu slice).  FP64 inputs.
 // rho2 = r^2 - Rg^2, rmu = r mu, delta = rmu^2 - rho2 (ray/ground discriminant).
 __device__ __forceinline__ void tex4d_rmu(const SetDesc &s, double rho2, double rmu,
-                                          double delta, double &tz, double &ty) {
+                                          double delta, double mu, double &tz, double &ty) {
     double rho = sqrt(fmax(rho2, 0.0));
     double f, c;
+    if (s.lin4) {   // common.glsl:164: uMu = 0.5/RES_MU + (mu + 1)/2 (1 - 1/RES_MU)
+        ty = (0.5 / s.RMU + (mu + 1.0) / 2.0 * (1.0 - 1.0 / s.RMU)) * s.RMU - 0.5;
+        tz = (0.5 / s.RR + rho / s.H * (1.0 - 1.0 / s.RR)) * s.RR - 0.5;
+        return;
+    }
     if (rmu < 0.0 && delta > 0.0) {
         // (rmu + sqrt(delta)) / rho == -rho / (sqrt(delta) - rmu), finite at rho = 0
         f = -rho / (sqrt(delta) - rmu);
@@ -209,13 +234,16 @@ __device__ __forceinline__ void tex4d_rmu(const SetDesc &s, double rho2, double 
 }
 
 __device__ __forceinline__ double tex4d_umus(const SetDesc &s, double muS) {
+    if (s.lin4) return 0.5 / s.RMUS + fmax(muS + 0.2, 0.0) / 1.2 * (1.0 - 1.0 / s.RMUS);   // :165
     return 0.5 / s.RMUS +
            (atan(fmax(muS, -0.1975) * tan(1.26 * 1.1)) / 1.1 + (1.0 - 0.26)) * 0.5 *
                (1.0 - 1.0 / s.RMUS);
 }
 
 // FP32 uMuS texel offset inside a slice: uMuS * RMUS - 0.5.
+template <bool LIN>   // the parameterisation fixed at compile time (inner loops)
 __device__ __forceinline__ float tex4d_txs_f(const SetDesc &s, float muS) {
+    if (LIN) return fmaxf(muS + 0.2f, 0.0f) * (1.0f / 1.2f) * (float)(s.RMUS - 1);
     float a = atanf(fmaxf(muS, -0.1975f) * s.f_tMuS) * (1.0f / 1.1f) + 0.74f;
     return a * 0.5f * (float)(s.RMUS - 1);
 }

@@ -196,13 +196,23 @@ size_t work_bytes(const atmo_params &p, const atmo_options &o) {
     return b;
 }
 
+// Flags that change what the tables hold (part of a plan's identity).
+constexpr int kTableFlags = ATMO_FLAG_HALF_STORAGE | ATMO_FLAG_LINEAR_TRANSMITTANCE | ATMO_FLAG_LINEAR_INSCATTER;
+bool lin_t(const atmo_options &o) { return (o.flags & ATMO_FLAG_LINEAR_TRANSMITTANCE) != 0; }
+bool lin_4(const atmo_options &o) { return (o.flags & ATMO_FLAG_LINEAR_INSCATTER) != 0; }
+void set_table_flags(SetDesc &s, const atmo_options &o) {
+    s.hs = (o.flags & ATMO_FLAG_HALF_STORAGE) ? 1 : 0;
+    s.linT = (o.flags & ATMO_FLAG_LINEAR_TRANSMITTANCE) ? 1 : 0;
+    s.lin4 = (o.flags & ATMO_FLAG_LINEAR_INSCATTER) ? 1 : 0;
+}
+
 bool same_params(const atmo_params &a, const atmo_params &b) { return memcmp(&a, &b, sizeof a) == 0; }
 bool same_opts(const atmo_options &a, const atmo_options &b) {
     return a.orders == b.orders && a.transmittance_samples == b.transmittance_samples &&
            a.inscatter_samples == b.inscatter_samples && a.irradiance_samples == b.irradiance_samples &&
            a.spherical_samples == b.spherical_samples && a.texels_per_thread == b.texels_per_thread &&
            a.inscatterN_form == b.inscatterN_form &&
-           (a.flags & ATMO_FLAG_HALF_STORAGE) == (b.flags & ATMO_FLAG_HALF_STORAGE);
+           (a.flags & kTableFlags) == (b.flags & kTableFlags);
 }
 
 struct DevBuf {
@@ -534,7 +544,7 @@ atmo_status launch_all(Plan &pl, hipStream_t st, PassTimer *tm = nullptr) {
     HIPCHK(launch_irradiance1(S, T + pl.off[P_E1], pl.cnt[P_E1], st));
     HIPCHK(t.end());
     HIPCHK(t.begin(2, 1));
-    HIPCHK(launch_inscatter1(S, T + pl.off[P_S1], pl.cnt[P_S1], o.inscatter_samples, st));
+    HIPCHK(launch_inscatter1(S, T + pl.off[P_S1], pl.cnt[P_S1], o.inscatter_samples, lin_t(o), st));
     HIPCHK(t.end());
     for (int order = 2; order <= o.orders; ++order) {
         const bool first = order == 2;
@@ -548,7 +558,8 @@ atmo_status launch_all(Plan &pl, hipStream_t st, PassTimer *tm = nullptr) {
         HIPCHK(t.end());
         HIPCHK(t.begin(5, order));
         HIPCHK(launch_inscatterN(S, T + pl.off[P_SN], pl.cnt[P_SN], o.inscatter_samples, pl.snCap, pl.snSb,
-                                 lds_inscatterN(sn_max_row_texels(pl.params, pl.snCap), pl.snSb), st));
+                                 lds_inscatterN(sn_max_row_texels(pl.params, pl.snCap), pl.snSb), lin_4(o),
+                                 st));
         HIPCHK(t.end());
     }
     return ATMO_OK;
@@ -671,7 +682,7 @@ atmo_status autotune(Plan &pl, hipStream_t st, Tuning &tu) {
             float ms = 0.f;
             atmo_status rc = time_launch(st, ms, [&]() {
                 return launch_inscatterN((const SetDesc *)ds.p, (const uint2 *)dt.p + off[P_SN], cnt[P_SN],
-                                         o.inscatter_samples, caps[ci], sbs[si], lds, st);
+                                         o.inscatter_samples, caps[ci], sbs[si], lds, lin_4(o), st);
             });
             if (rc != ATMO_OK) return rc;
             tu.msN[si * 9 + ci * 3 + xo] = ms;
@@ -744,7 +755,7 @@ atmo_status build_plan(atmo_engine *e, const atmo_params *sets, int nsets, const
         s.dSM4 = (float4 *)b; b += align256(z.n4 * 16);
         s.dJ4 = o.orders >= 2 ? (float4 *)b : nullptr;
         s.dJb = 12;  // the ray gathers of dSn read 12 bytes per texel
-        s.hs = (o.flags & ATMO_FLAG_HALF_STORAGE) ? 1 : 0;
+        set_table_flags(s, o);
         s.layers = (const LayerGeom *)pl->dlayers.p + loff[i];
     }
     HIPCHK(pl->dsets.ensure(nsets * sizeof(SetDesc)));
@@ -1023,7 +1034,7 @@ static atmo_status pass_setup(atmo_engine *e, const atmo_params *p, const atmo_o
     if ((st = validate(*p, 0)) != ATMO_OK) return st;
     HIPCHK(hipSetDevice(e->device));
     fill_desc(s, *p);
-    s.hs = (oo.flags & ATMO_FLAG_HALF_STORAGE) ? 1 : 0;
+    set_table_flags(s, oo);
     std::vector<LayerGeom> layers;
     for (int l = 0; l < p->RES_R; ++l) layers.push_back(layer_geom(*p, l));
     HIPCHK(e->pass_layers.ensure(layers.size() * sizeof(LayerGeom)));
@@ -1055,7 +1066,7 @@ static atmo_status pass_run(atmo_engine *e, const atmo_params *p, const atmo_opt
     switch (pass) {
     case P_T: HIPCHK(launch_transmittance(S, T + off[P_T], cnt[P_T], o.transmittance_samples, hs)); break;
     case P_E1: HIPCHK(launch_irradiance1(S, T + off[P_E1], cnt[P_E1], hs)); break;
-    case P_S1: HIPCHK(launch_inscatter1(S, T + off[P_S1], cnt[P_S1], o.inscatter_samples, hs)); break;
+    case P_S1: HIPCHK(launch_inscatter1(S, T + off[P_S1], cnt[P_S1], o.inscatter_samples, lin_t(o), hs)); break;
     case P_J: {
         const int ch = chunk_inscatterS(o.spherical_samples, p->RES_NU, p->SKY_W, first);
         if (!ch) return fail(ATMO_ERR_UNSUPPORTED, "SKY_W %d: the ground irradiance row does not fit in LDS", p->SKY_W);
@@ -1075,7 +1086,7 @@ static atmo_status pass_run(atmo_engine *e, const atmo_params *p, const atmo_opt
     case P_SN: {
         const int sb = form_sb(o.inscatterN_form);
         const size_t lds = lds_inscatterN(sn_max_row_texels(std::vector<atmo_params>(1, *p), 4), sb);
-        HIPCHK(launch_inscatterN(S, T + off[P_SN], cnt[P_SN], o.inscatter_samples, 4, sb, lds, hs));
+        HIPCHK(launch_inscatterN(S, T + off[P_SN], cnt[P_SN], o.inscatter_samples, 4, sb, lds, lin_4(o), hs));
         break;
     }
     }

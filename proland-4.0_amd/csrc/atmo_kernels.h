@@ -29,8 +29,9 @@ int chunk_irradianceN(int NI, int RNU, bool first);
 hipError_t launch_transmittance(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
                                 hipStream_t st);
 hipError_t launch_irradiance1(const SetDesc *sets, const uint2 *tiles, int ntiles, hipStream_t st);
+// linT / lin4: the sets' parameterisation flags (one per batch), compiled in
 hipError_t launch_inscatter1(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
-                             hipStream_t st);
+                             bool linT, hipStream_t st);
 // G: texels per thread (2, 3, 4 or 8); CH: theta rows per staging round
 hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS, int CH,
                              const void *dirT, const void *dirW, bool first, size_t lds, int G,
@@ -44,7 +45,7 @@ hipError_t launch_irradianceN(const SetDesc *sets, const uint2 *tiles, int ntile
 size_t lds_inscatterN(int max_row_texels, int sb);
 size_t static_lds_inscatterN();   // bytes of k_inscatterN's static LDS
 hipError_t launch_inscatterN(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
-                             int rpw_cap, int sb, size_t lds, hipStream_t st);
+                             int rpw_cap, int sb, size_t lds, bool lin4, hipStream_t st);
 hipError_t launch_check(const SetDesc *sets, const uint2 *tiles, int ntiles, unsigned int *flags,
                         hipStream_t st);
 

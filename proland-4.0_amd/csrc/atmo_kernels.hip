@@ -25,8 +25,14 @@ __global__ __launch_bounds__(256) void k_transmittance(const SetDesc *__restrict
     // getTransmittanceRMu (common.glsl:126-136), gl_FragCoord = (i + .5, j + .5)
     double rr = (j + 0.5) / s.TH;
     double m = (i + 0.5) / s.TW;
-    double r = s.Rg + rr * rr * (s.Rt - s.Rg);
-    double mu = -0.15 + tan(1.5 * m) / tan(1.5) * (1.0 + 0.15);
+    double r, mu;
+    if (s.linT) {
+        r = s.Rg + rr * (s.Rt - s.Rg);
+        mu = -0.15 + m * (1.0 + 0.15);
+    } else {
+        r = s.Rg + rr * rr * (s.Rt - s.Rg);
+        mu = -0.15 + tan(1.5 * m) / tan(1.5) * (1.0 + 0.15);
+    }
     double dx = limit_d(s, r, mu) / nsamp;
     double sR = 0.0, sM = 0.0;
     const double iHR = 1.0 / s.HR, iHM = 1.0 / s.HM;
@@ -118,6 +124,7 @@ struct S1Entry {
 
 constexpr int kS1Chunk = 64;
 
+template <bool LINT>   // linear transmittance parameterisation (the per-sample T lookup)
 __global__ __launch_bounds__(256) void k_inscatter1(const SetDesc *__restrict__ sets,
                                                     const uint2 *__restrict__ tiles,
                                                     int nsamp) {
@@ -192,7 +199,7 @@ __global__ __launch_bounds__(256) void k_inscatter1(const SetDesc *__restrict__ 
             en.eM = __expf(-hc * iHM);
             // r axis of T(max(Rg, ri), muSi): common.glsl:114-124
             int j0, j1;
-            gl_axis(sqrtf(hc * s.f_inv_dRt) * s.TH - 0.5f, s.TH, j0, j1, en.b);
+            gl_axis(T_ur(s, hc) * s.TH - 0.5f, s.TH, j0, j1, en.b);
             en.j0 = j0 * s.TW;
             en.j1 = j1 * s.TW;
             en.pad = 0;
@@ -215,7 +222,7 @@ __global__ __launch_bounds__(256) void k_inscatter1(const SetDesc *__restrict__ 
                 const float muSi = (float)num * en.iri;
                 int i0, i1;
                 float a;
-                gl_axis(atanf((muSi + 0.15f) * s.f_tMu) * (1.0f / 1.5f) * s.TW - 0.5f, s.TW, i0, i1, a);
+                gl_axis(T_umu_t<LINT>(s, muSi) * s.TW - 0.5f, s.TW, i0, i1, a);
                 const float3 t00 = ld3(T4 + en.j0 + i0), t01 = ld3(T4 + en.j0 + i1);
                 const float3 t10 = ld3(T4 + en.j1 + i0), t11 = ld3(T4 + en.j1 + i1);
                 const float3 Ts = lerp3(lerp3(t00, t01, a), lerp3(t10, t11, a), en.b);
@@ -370,7 +377,7 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
             const double ct = cos((it + 0.5) * (kPI / NS));
             const double rmu = r * ct;
             double tz, ty;
-            tex4d_rmu(s, rho2, rmu, rmu * rmu - rho2, tz, ty);
+            tex4d_rmu(s, rho2, rmu, rmu * rmu - rho2, ct, tz, ty);
             float4 v[NT];
             stage_slice<NT>(s, tabs, tz, ty, txs, kk - 1, v);
             if (FIRST) {  // fold the phase-function constants in
@@ -647,7 +654,7 @@ __global__ __launch_bounds__(256) void k_irradianceN(const SetDesc *__restrict__
             const double ct = cos((it + 0.5) * (kPI / NI));
             const double rmu = r * ct;
             double tz, ty;
-            tex4d_rmu(s, rho2, rmu, rmu * rmu - rho2, tz, ty);
+            tex4d_rmu(s, rho2, rmu, rmu * rmu - rho2, ct, tz, ty);
             float4 v[NT];
             stage_slice<NT>(s, tabs, tz, ty, txs, kk - 1, v);
             if (FIRST) {
@@ -763,7 +770,7 @@ struct SnEntry {
 // neighbours of B_i and lerps in x.  Both compute the same trilinear filter;
 // they differ only in FP32 rounding order.  `sb` samples are blended per LDS
 // round (dynamic LDS: sb * rows * RNU*RMU_S float4).
-template <bool STAGED>
+template <bool STAGED, bool LIN4>   // LIN4: linear inscatter parameterisation (per-sample mu_s)
 __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ sets,
                                                     const uint2 *__restrict__ tiles,
                                                     int nsamp, int rpw_cap, int sb) {
@@ -845,7 +852,8 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
                 f = (sqrt(fmax(delta + s.H * s.H, 0.0)) - rmui) / (rho + s.H);
                 c = 0.5 + 0.5 / s.RMU;
             }
-            const double ty = (c + f * (0.5 - 1.0 / s.RMU)) * s.RMU - 0.5;
+            const double ty = s.lin4 ? (0.5 / s.RMU + (rmui / ri + 1.0) / 2.0 * (1.0 - 1.0 / s.RMU)) * s.RMU - 0.5
+                                     : (c + f * (0.5 - 1.0 / s.RMU)) * s.RMU - 0.5;
             const double tz = rho / s.H * (s.RR - 1.0);
             int z0, z1, y0, y1;
             float az, ay;
@@ -885,7 +893,7 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
                     const SnEntry en = er[js];
                     const float muSi = fmaf(nuf[j], en.tw.w, muSr[j]) * en.ir.x;
                     // x: texel column; clamped to [0, W4-1] and split so that i0 + 1 <= W4 - 1
-                    float tx = __builtin_amdgcn_fmed3f(kbase + tex4d_txs_f(s, muSi), 0.0f, (float)W4m1);
+                    float tx = __builtin_amdgcn_fmed3f(kbase + tex4d_txs_f<LIN4>(s, muSi), 0.0f, (float)W4m1);
                     const unsigned i0 = min(__float2uint_rz(tx), W4m2);
                     const float ax = tx - (float)i0;
                     float3 v = make_float3(0.f, 0.f, 0.f);
@@ -963,7 +971,7 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
                     for (int k = 0; k < ns; ++k) {
                         const SnEntry &en = er[s0 + k];
                         const float muSi = fmaf(nuf[j], en.tw.w, muSr[j]) * en.ir.x;
-                        float tx = __builtin_amdgcn_fmed3f(kbase + tex4d_txs_f(s, muSi), 0.0f, (float)W4m1);
+                        float tx = __builtin_amdgcn_fmed3f(kbase + tex4d_txs_f<LIN4>(s, muSi), 0.0f, (float)W4m1);
                         const unsigned i0 = min(__float2uint_rz(tx), W4m2);
                         const float ax = tx - (float)i0;
                         const float4 *B = blend + (size_t)k * per + rowbase;
@@ -1076,9 +1084,12 @@ hipError_t launch_irradiance1(const SetDesc *sets, const uint2 *tiles, int ntile
     return hipGetLastError();
 }
 hipError_t launch_inscatter1(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
-                             hipStream_t st) {
+                             bool linT, hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_inscatter1, dim3(ntiles), dim3(256), 0, st, sets, tiles, nsamp);
+    if (linT)
+        hipLaunchKernelGGL(k_inscatter1<true>, dim3(ntiles), dim3(256), 0, st, sets, tiles, nsamp);
+    else
+        hipLaunchKernelGGL(k_inscatter1<false>, dim3(ntiles), dim3(256), 0, st, sets, tiles, nsamp);
     return hipGetLastError();
 }
 hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS, int CH,
@@ -1123,12 +1134,16 @@ size_t lds_inscatterN(int max_row_texels, int sb) {
     return sb > 0 ? sizeof(float4) * (size_t)sb * (size_t)max_row_texels : 0;
 }
 hipError_t launch_inscatterN(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
-                             int rpw_cap, int sb, size_t lds, hipStream_t st) {
+                             int rpw_cap, int sb, size_t lds, bool lin4, hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
-    if (sb > 0)
-        hipLaunchKernelGGL(k_inscatterN<true>, dim3(ntiles), dim3(256), lds, st, sets, tiles, nsamp, rpw_cap, sb);
-    else
-        hipLaunchKernelGGL(k_inscatterN<false>, dim3(ntiles), dim3(256), 0, st, sets, tiles, nsamp, rpw_cap, 0);
+#define ATMO_LN(S, L, LDS, SB) \
+    hipLaunchKernelGGL((k_inscatterN<S, L>), dim3(ntiles), dim3(256), LDS, st, sets, tiles, nsamp, rpw_cap, SB)
+    if (sb > 0) {
+        if (lin4) ATMO_LN(true, true, lds, sb); else ATMO_LN(true, false, lds, sb);
+    } else {
+        if (lin4) ATMO_LN(false, true, 0, 0); else ATMO_LN(false, false, 0, 0);
+    }
+#undef ATMO_LN
     return hipGetLastError();
 }
 hipError_t launch_check(const SetDesc *sets, const uint2 *tiles, int ntiles, unsigned int *flags,

@@ -351,8 +351,10 @@ atmo_status atmo_surrogate_build_dataset(atmo_surrogate *s, const atmo_params *s
     if (!s || !sets || nsets <= 0 || !opts || samples_per_set <= 0)
         return fail(ATMO_ERR_INVALID_ARGUMENT, "bad arguments");
     if (opts->orders < 2) return fail(ATMO_ERR_INVALID_ARGUMENT, "training data needs orders >= 2");
-    if (opts->flags & ATMO_FLAG_HALF_STORAGE)
-        return fail(ATMO_ERR_INVALID_ARGUMENT, "the surrogate works on FP32 tables (no ATMO_FLAG_HALF_STORAGE)");
+    if (opts->flags & (ATMO_FLAG_HALF_STORAGE | ATMO_FLAG_LINEAR_TRANSMITTANCE | ATMO_FLAG_LINEAR_INSCATTER))
+        return fail(ATMO_ERR_INVALID_ARGUMENT,
+                    "the surrogate works on FP32 tables of the non-linear parameterisation "
+                    "(no ATMO_FLAG_HALF_STORAGE, ATMO_FLAG_LINEAR_*)");
     HIPCHK(hipSetDevice(s->device));
     size_t rows = 0;
     for (int i = 0; i < nsets; ++i) rows += (size_t)samples_per_set + (size_t)sets[i].SKY_W * sets[i].SKY_H;
@@ -475,8 +477,10 @@ atmo_status atmo_surrogate_precompute(atmo_surrogate *s, const atmo_params *sets
     HIPCHK(hipSetDevice(s->device));
     atmo_options o = opts ? *opts : atmo_options{};
     if (!opts) atmo_options_default(&o);
-    if (o.flags & ATMO_FLAG_HALF_STORAGE)
-        return fail(ATMO_ERR_INVALID_ARGUMENT, "the surrogate works on FP32 tables (no ATMO_FLAG_HALF_STORAGE)");
+    if (o.flags & (ATMO_FLAG_HALF_STORAGE | ATMO_FLAG_LINEAR_TRANSMITTANCE | ATMO_FLAG_LINEAR_INSCATTER))
+        return fail(ATMO_ERR_INVALID_ARGUMENT,
+                    "the surrogate works on FP32 tables of the non-linear parameterisation "
+                    "(no ATMO_FLAG_HALF_STORAGE, ATMO_FLAG_LINEAR_*)");
     atmo_options o1 = o;
     o1.orders = 1;
     hipStream_t hs = (hipStream_t)stream;

@@ -73,6 +73,8 @@ FLAG_NO_GRAPH = 2
 FLAG_TIME_PASSES = 4
 FLAG_NO_AUTOTUNE = 8
 FLAG_HALF_STORAGE = 16
+FLAG_LINEAR_TRANSMITTANCE = 32
+FLAG_LINEAR_INSCATTER = 64
 
 
 def earth(**over):

@@ -16,8 +16,9 @@ torch = pytest.importorskip("torch")
 from oracle import oracle as O  # noqa: E402
 from prolandatmo import lib as L  # noqa: E402
 from prolandatmo.metrics import rel_err, summary  # noqa: E402
-from prolandatmo.params import (AtmoParams, FLAG_HALF_STORAGE, SEED_BATCH, draw_sets,  # noqa: E402
-                                earth, options, with_res)
+from prolandatmo.params import (AtmoParams, FLAG_HALF_STORAGE, FLAG_LINEAR_INSCATTER,  # noqa: E402
+                                FLAG_LINEAR_TRANSMITTANCE, SEED_BATCH, draw_sets, earth, options,
+                                with_res)
 
 pytestmark = pytest.mark.gpu
 
@@ -173,6 +174,23 @@ def test_half_storage_matches_oracle(eng, ci):
             assert sm["max"] <= 1e-2, (name, sm)
         else:
             assert steps <= 4.0, (name, steps)
+
+
+@pytest.mark.parametrize("which", ["transmittance", "inscatter", "both"])
+def test_linear_parameterisations_match_oracle(eng, which):
+    """The reference built without TRANSMITTANCE_NON_LINEAR and/or
+    INSCATTER_NON_LINEAR (common.glsl:105-106): ATMO_FLAG_LINEAR_* against the
+    oracle's linear branches, whole chain, both k_inscatterN forms."""
+    flags = {"transmittance": FLAG_LINEAR_TRANSMITTANCE, "inscatter": FLAG_LINEAR_INSCATTER,
+             "both": FLAG_LINEAR_TRANSMITTANCE | FLAG_LINEAR_INSCATTER}[which]
+    for ci in (0, 2):
+        p = CASES[ci]
+        oT, oE, oS = O.precompute(p, options(flags=flags))
+        for form in (1, 2):
+            T, E, S = _chain(eng, p, options(flags=flags | 8, inscatterN_form=form))
+            _assert("T", T, oT, TOL)
+            _assert("E", E, oE, TOL)
+            _assert("S", S, oS, TOL)
 
 
 def test_half_storage_is_part_of_the_plan(eng):
