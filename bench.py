@@ -179,7 +179,7 @@ def main():
     # form blends the 4 dJ rows around the sample (2 layers x 2 mu rows,
     # RES_NU*RES_MU_S texels of 12 bytes each) into LDS.
     tu = eng.tuning()
-    sn_name = "k_inscatterN<%s>" % ("true" if tu.get("inscatterN_staged_samples", 0) else "false")
+    sn_name = "k_inscatterN<%s,false>" % ("true" if tu.get("inscatterN_staged_samples", 0) else "false")
     sn_ms = [ms for name, order, ms in times if name == "inscatterN"]
     sn_avg = sum(sn_ms) / len(sn_ms)
     sn_traffic = profiled(TRAFFIC_PROFILE, "traffic", sn_name)
