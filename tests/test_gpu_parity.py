@@ -287,6 +287,17 @@ def test_direction_grids_staged_in_rounds(eng, case):
     _assert("S", S, oS, TOL)
 
 
+def test_smallest_tables_the_abi_accepts(eng):
+    """RES_R 2, RES_MU 4, RES_MU_S 2, RES_NU 2, transmittance and irradiance
+    2x2: every (size - 1) divisor of the parameterisations is 1."""
+    p = small(res=(2, 4, 2, 2), T=(2, 2), E=(2, 2))
+    T, E, S = _chain(eng, p)
+    oT, oE, oS = O.precompute(p, options())
+    _assert("T", T, oT, TOL)
+    _assert("E", E, oE, TOL)
+    _assert("S", S, oS, TOL)
+
+
 def test_golden_tiny_fixtures(eng):
     g = np.load(os.path.join(GOLDEN, "tiny_earth.npz"))
     p = _params(g["params"])
