@@ -372,6 +372,50 @@ def test_mixed_resolution_batch_equals_single_runs(eng):
         assert np.array_equal(S.double().cpu().numpy(), S1)
 
 
+@pytest.mark.parametrize("flags", [8, 0])
+def test_two_engines_on_two_threads(flags):
+    """Two engines driven from two host threads at once, each on a stream of its
+    own (the C ABI releases the GIL): per-engine plans, capture streams and
+    graphs, and the process-wide tuning cache under its lock.  Default variants
+    (flags 8): bit-identical to a sequential run; autotuned: within 1e-5."""
+    import threading
+    cases = {"a": small(), "b": small(res=(6, 16, 6, 4), T=(32, 8), E=(8, 4))}
+    ref = {}
+    for name, p in cases.items():
+        e = L.Engine(0)
+        ref[name] = [t.double().cpu().numpy() for t in e.precompute(p, options(flags=8))[0]]
+        torch.cuda.synchronize()
+        e.close()
+    res, errs = {}, []
+
+    def work(name, p):
+        try:
+            e = L.Engine(0)
+            st = torch.cuda.Stream()
+            outs = L.alloc_outputs(p)
+            for _ in range(3):
+                e.precompute(p, options(flags=flags), outs=[outs], stream=st)
+                st.synchronize()
+            res[name] = [t.double().cpu().numpy() for t in outs]
+            e.close()
+        except Exception as ex:   # reported by the main thread
+            errs.append((name, ex))
+
+    th = [threading.Thread(target=work, args=(n, p)) for n, p in cases.items()]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in th)
+    assert not errs, errs
+    for name in cases:
+        for x, y in zip(res[name], ref[name]):
+            if flags == 8:
+                assert np.array_equal(x, y), name
+            else:
+                _assert("thread " + name, x, y, 1e-5)
+
+
 def test_graph_replays_match_eager(eng):
     p = small()
     eager = _chain(eng, p, options(flags=2))
