@@ -203,7 +203,9 @@ __device__ __forceinline__ void mu_mus_nu(const SetDesc &s, const LayerGeom &L, 
 // muS of a texel column (depends on x only).
 __device__ __forceinline__ double texel_mus(const SetDesc &s, int xm) {
     double m = xm / (s.RMUS - 1.0);
-    if (s.lin4) return -0.2 + m * 1.2;
+    // linear: -0.2 + m 1.2 rounded twice, never fused (a column with muS = 0
+    // in exact arithmetic keeps the sign of the two-step rounding)
+    if (s.lin4) return __dadd_rn(-0.2, __dmul_rn(m, 1.2));
     return tan((2.0 * m - 1.0 + 0.26) * 1.1) / tan(1.26 * 1.1);
 }
 

@@ -237,6 +237,47 @@ def test_batch_of_seeded_and_extreme_atmospheres_matches_oracle(eng):
         _assert("S[%d]" % i, S.double().cpu().numpy(), oS, TOL)
 
 
+def _random_case(rng):
+    """A seeded atmosphere of the batch distribution at a random small
+    resolution (odd sizes included) with random options."""
+    R = int(rng.integers(2, 7))
+    MU = 2 * int(rng.integers(2, 13))
+    MUS = int(rng.integers(2, 9))
+    NU = int(rng.integers(2, 9))
+    p = seeded(1, (R, MU, MUS, NU), seed=int(rng.integers(1, 2 ** 31)))[0]
+    p.TRANSMITTANCE_W, p.TRANSMITTANCE_H = int(rng.integers(2, 80)), int(rng.integers(2, 24))
+    p.SKY_W, p.SKY_H = int(rng.integers(2, 24)), int(rng.integers(2, 10))
+    o = options(orders=int(rng.integers(1, 6)), transmittance_samples=int(rng.integers(20, 600)),
+                inscatter_samples=int(rng.integers(5, 80)),
+                irradiance_samples=2 * int(rng.integers(2, 24)),
+                spherical_samples=int(rng.integers(4, 24)),
+                flags=int(rng.choice([0, FLAG_LINEAR_TRANSMITTANCE, FLAG_LINEAR_INSCATTER])))
+    if (o.flags & FLAG_LINEAR_INSCATTER) and (p.RES_MU_S - 1) % 6 == 0:
+        # the linear mu_s grid -0.2 + 1.2 m/(RES_MU_S - 1) then has a column at
+        # mu_s = 0: the sun on the horizon, where the sun test of a ray's ground
+        # sample (inscatter1.glsl:107-110) compares 0 with 0 and rounding decides
+        p.RES_MU_S += 1
+    return p, o
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_resolutions_and_options_match_oracle(eng, seed):
+    """Six random cases per seed (resolutions, table sizes, orders, sample
+    counts, parameterisation), each run alone and compared with the oracle."""
+    rng = np.random.default_rng(1000 + seed)
+    for case in range(6):
+        p, o = _random_case(rng)
+        print("case", seed, case, (p.RES_R, p.RES_MU, p.RES_MU_S, p.RES_NU),
+              (p.TRANSMITTANCE_W, p.TRANSMITTANCE_H, p.SKY_W, p.SKY_H), o.orders,
+              o.transmittance_samples, o.inscatter_samples, o.irradiance_samples,
+              o.spherical_samples, o.flags, flush=True)
+        T, E, S = _chain(eng, p, o)
+        oT, oE, oS = O.precompute(p, o)
+        _assert("T", T, oT, TOL)
+        _assert("E", E, oE, TOL)
+        _assert("S", S, oS, TOL)
+
+
 def test_chain_orders_1_and_6(eng):
     p = small()
     for n in (1, 6):
