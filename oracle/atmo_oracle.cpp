@@ -26,6 +26,9 @@
  *   D3 getTransmittanceUV: sqrt(max(r - Rg, 0) / (Rt - Rg)).
  *   D4 copy passes read the texel they sample at its centre directly (in exact
  *      arithmetic a LINEAR fetch at a texel centre is that texel).
+ *   D5 the transmittance ratios min(T(x)/T(y), 1) (common.glsl:272-282 and the
+ *      consumer's) take min as GPUs execute GLSL min, IEEE minNum: a 0/0 ratio
+ *      (both lookups 0: FP16 storage, or T underflowing) gives 1.
  * Uniforms are FP32 in the reference; the oracle takes the FP32 parameter
  * values and the FP32-rounded per-layer r and dhdH (PreprocessAtmo.cpp:255-271
  * computes them in double and uploads floats) and does everything else in FP64.
@@ -214,7 +217,10 @@ void T_segment(const P &p, const double *T, double r, double mu, double d, doubl
         T_lookup(p, T, r1, -mu1, a);
         T_lookup(p, T, r, -mu, b);
     }
-    for (int c = 0; c < 3; ++c) out[c] = std::min(a[c] / b[c], 1.0);
+    /* GLSL min on a GPU is IEEE minNum (the non-NaN operand): where both
+     * lookups are 0 (FP16 storage, or T underflowing far below the horizon)
+     * the ratio 0/0 gives 1, not NaN (D5) */
+    for (int c = 0; c < 3; ++c) out[c] = std::fmin(a[c] / b[c], 1.0);
 }
 
 /* common.glsl:138-142 + 284-287 irradiance(sampler, r, muS) */
@@ -811,7 +817,7 @@ static void transmittance_ratio(const P &p, const double *T, double r0, double m
         T_lookup(p, T, r, -mu, a);
         T_lookup(p, T, r0, -mu0, b);
     }
-    for (int c = 0; c < 3; ++c) out[c] = std::min(a[c] / b[c], 1.0);
+    for (int c = 0; c < 3; ++c) out[c] = std::fmin(a[c] / b[c], 1.0);   /* D5 */
 }
 
 /* :371-418 skyRadiance */
@@ -906,12 +912,12 @@ void oracle_in_scattering(const atmo_params *a, const double *T, const double *S
             double x[3], y[3];
             T_lookup(p, T, r, mu, x);
             T_lookup(p, T, r1, mu1, y);
-            for (int c = 0; c < 3; ++c) ext[c] = std::min(x[c] / y[c], 1.0);
+            for (int c = 0; c < 3; ++c) ext[c] = std::fmin(x[c] / y[c], 1.0);   /* D5 */
         } else {
             double x[3], y[3];
             T_lookup(p, T, r1, -mu1, x);
             T_lookup(p, T, r, -mu, y);
-            for (int c = 0; c < 3; ++c) ext[c] = std::min(x[c] / y[c], 1.0);
+            for (int c = 0; c < 3; ++c) ext[c] = std::fmin(x[c] / y[c], 1.0);   /* D5 */
         }
         auto diff = [&](double m, double rr1, double mm1, double out[4]) {
             double s0[4], s1[4];

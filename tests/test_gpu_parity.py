@@ -278,6 +278,30 @@ def test_random_resolutions_and_options_match_oracle(eng, seed):
         _assert("S", S, oS, TOL)
 
 
+@pytest.mark.parametrize("seed", range(3))
+def test_random_mixed_batches_match_oracle(eng, seed):
+    """Batches of five random sets of different resolutions sharing one random
+    options draw (seed 2: FP16 storage as well), every set against the oracle."""
+    rng = np.random.default_rng(2000 + seed)
+    cases = [_random_case(rng) for _ in range(5)]
+    o = cases[0][1]
+    if seed == 2:
+        o.flags |= FLAG_HALF_STORAGE
+    sets = [c[0] for c in cases]
+    if o.flags & FLAG_LINEAR_INSCATTER:
+        for p in sets:
+            if (p.RES_MU_S - 1) % 6 == 0:
+                p.RES_MU_S += 1   # no exact-horizon column (see _random_case)
+    outs = eng.precompute(sets, o)
+    torch.cuda.synchronize()
+    for i, (p, (T, E, S)) in enumerate(zip(sets, outs)):
+        oT, oE, oS = O.precompute(p, o)
+        tol = 1e-2 if o.flags & FLAG_HALF_STORAGE else TOL   # FP16 bar of test_half_storage_matches_oracle
+        _assert("T[%d]" % i, T.double().cpu().numpy(), oT, tol)
+        _assert("E[%d]" % i, E.double().cpu().numpy(), oE, tol)
+        _assert("S[%d]" % i, S.double().cpu().numpy(), oS, tol)
+
+
 def test_chain_orders_1_and_6(eng):
     p = small()
     for n in (1, 6):
