@@ -48,6 +48,12 @@ struct SetDesc {
     int n4;                         // RR * RMU * W4
     // tables (device pointers)
     float4 *T4, *dE4, *E4, *dSR4, *dSM4, *dJ4, *S4;
+    float4 *tau4;                   // optical depth -ln T of the T texels (rgb), or null
+    // transmittances along the view ray of each (layer, mu row) at the inscatter
+    // samples, [RR][RMU][nsamp + 1], and from the ground point to each layer
+    // radius per theta row of the spherical integral, [RR][NS]; null: computed
+    // inline from T (pass-level entry points)
+    float4 *tview, *tground;
     int dJb;                        // bytes per dJ texel: 16 (float4) or 12 (packed rgb)
     int hs;                         // 1: table values rounded to FP16 on store (q16)
     int linT, lin4;                 // 1: the linear transmittance / inscatter parameterisation
@@ -147,6 +153,62 @@ __device__ __forceinline__ float T_umu(const SetDesc &s, float mu) {
 // common.glsl:223-226: T(r, mu), bilinear.
 __device__ __forceinline__ float3 T_lookup(const SetDesc &s, float h, float mu) {
     return bilinear(s.T4, s.TW, s.TH, T_umu(s, mu) * s.TW - 0.5f, T_ur(s, h) * s.TH - 0.5f);
+}
+
+// T(x) / T(y), min 1 (the segment transmittance of common.glsl:272-282) for
+// the bilinear lookups T(x) = T_lookup(s, hx, mux), T(y) = T_lookup(s, hy, muy).
+// With the optical-depth table it is evaluated as
+//   T(x) = e^-m sum_i w_i e^(m - tau_i),  m = the smallest depth among the taps
+//   of nonzero weight,
+// which equals the bilinear lookup in exact arithmetic but cannot underflow:
+// two transmittances far below FP32's range (long grazing paths through dense
+// or large atmospheres) still give their finite ratio.  Without the table, or
+// with FP16 storage (which keeps the reference's stored T), the plain quotient.
+__device__ __forceinline__ void T_taps(const SetDesc &s, float h, float mu, int idx[4], float w[4]) {
+    int i0, i1, j0, j1;
+    float a, b;
+    gl_axis(T_umu(s, mu) * s.TW - 0.5f, s.TW, i0, i1, a);
+    gl_axis(T_ur(s, h) * s.TH - 0.5f, s.TH, j0, j1, b);
+    idx[0] = j0 * s.TW + i0; idx[1] = j0 * s.TW + i1;
+    idx[2] = j1 * s.TW + i0; idx[3] = j1 * s.TW + i1;
+    w[0] = (1.0f - a) * (1.0f - b); w[1] = a * (1.0f - b);
+    w[2] = (1.0f - a) * b; w[3] = a * b;
+}
+// ln T = -m + ln S of one channel c of a lookup with taps idx, weights w
+__device__ __forceinline__ void T_log_channel(const float *tau, const int idx[4], const float w[4],
+                                              int c, float &m, float &S) {
+    float t[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) t[q] = tau[4 * idx[q] + c];
+    m = __builtin_huge_valf();
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (w[q] > 0.0f) m = fminf(m, t[q]);
+    S = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (w[q] > 0.0f) S = fmaf(w[q], expf(m - t[q]), S);
+}
+__device__ __forceinline__ float3 T_ratio_lin(const SetDesc &s, float hx, float mux, float hy, float muy) {
+    const float3 a = T_lookup(s, hx, mux), b = T_lookup(s, hy, muy);
+    return make_float3(fminf(a.x / b.x, 1.f), fminf(a.y / b.y, 1.f), fminf(a.z / b.z, 1.f));
+}
+__device__ __forceinline__ float3 T_ratio(const SetDesc &s, float hx, float mux, float hy, float muy) {
+    if (!s.tau4 || s.hs) return T_ratio_lin(s, hx, mux, hy, muy);
+    int ix[4], iy[4];
+    float wx[4], wy[4], r[3];
+    T_taps(s, hx, mux, ix, wx);
+    T_taps(s, hy, muy, iy, wy);
+    const float *tau = (const float *)s.tau4;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float mx, Sx, my, Sy;
+        T_log_channel(tau, ix, wx, c, mx, Sx);
+        T_log_channel(tau, iy, wy, c, my, Sy);
+        const float lr = (my - mx) + logf(Sx / Sy);
+        r[c] = lr >= 0.0f ? 1.0f : expf(lr);
+    }
+    return make_float3(r[0], r[1], r[2]);
 }
 
 // Phase functions, common.glsl:290-297.

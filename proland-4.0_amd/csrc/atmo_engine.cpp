@@ -191,8 +191,12 @@ SetSizes sizes(const atmo_params &p) {
 // work tables per set: T4, dE4, E4, dSR4, dSM4, dJ4
 size_t work_bytes(const atmo_params &p, const atmo_options &o) {
     SetSizes z = sizes(p);
-    size_t b = align256(z.nT * 16) + 2 * align256(z.nE * 16) + 2 * align256(z.n4 * 16);
-    if (o.orders >= 2) b += align256(z.n4 * 12);  // dJ, packed rgb
+    size_t b = 2 * align256(z.nT * 16) + 2 * align256(z.nE * 16) + 2 * align256(z.n4 * 16);   // T, tau, ...
+    b += align256((size_t)p.RES_R * p.RES_MU * (o.inscatter_samples + 1) * 16);   // tview
+    if (o.orders >= 2) {
+        b += align256(z.n4 * 12);                                         // dJ, packed rgb
+        b += align256((size_t)p.RES_R * o.spherical_samples * 16);        // tground
+    }
     return b;
 }
 
@@ -539,6 +543,9 @@ atmo_status launch_all(Plan &pl, hipStream_t st, PassTimer *tm = nullptr) {
     const char *dirs = (const char *)pl.ddirs.p;
     HIPCHK(t.begin(0, 1));
     HIPCHK(launch_transmittance(S, T + pl.off[P_T], pl.cnt[P_T], o.transmittance_samples, st));
+    // the transmittance tables of the later passes (timed with the T pass)
+    HIPCHK(launch_tview(S, T + pl.off[P_S1], pl.cnt[P_S1], o.inscatter_samples, st));
+    if (o.orders >= 2) HIPCHK(launch_tground(S, T + pl.off[P_J], pl.cnt[P_J], o.spherical_samples, st));
     HIPCHK(t.end());
     HIPCHK(t.begin(1, 1));
     HIPCHK(launch_irradiance1(S, T + pl.off[P_E1], pl.cnt[P_E1], st));
@@ -749,11 +756,17 @@ atmo_status build_plan(atmo_engine *e, const atmo_params *sets, int nsets, const
         SetSizes z = sizes(sets[i]);
         char *b = ar + base[i];
         s.T4 = (float4 *)b; b += align256(z.nT * 16);
+        s.tau4 = (float4 *)b; b += align256(z.nT * 16);
+        s.tview = (float4 *)b; b += align256((size_t)sets[i].RES_R * sets[i].RES_MU * (o.inscatter_samples + 1) * 16);
         s.dE4 = (float4 *)b; b += align256(z.nE * 16);
         s.E4 = (float4 *)b; b += align256(z.nE * 16);
         s.dSR4 = (float4 *)b; b += align256(z.n4 * 16);
         s.dSM4 = (float4 *)b; b += align256(z.n4 * 16);
         s.dJ4 = o.orders >= 2 ? (float4 *)b : nullptr;
+        if (o.orders >= 2) {
+            b += align256(z.n4 * 12);
+            s.tground = (float4 *)b;
+        }
         s.dJb = 12;  // the ray gathers of dSn read 12 bytes per texel
         set_table_flags(s, o);
         s.layers = (const LayerGeom *)pl->dlayers.p + loff[i];

@@ -29,6 +29,10 @@ int chunk_irradianceN(int NI, int RNU, bool first);
 hipError_t launch_transmittance(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
                                 hipStream_t st);
 hipError_t launch_irradiance1(const SetDesc *sets, const uint2 *tiles, int ntiles, hipStream_t st);
+// transmittance tables of a precompute (SetDesc::tview on the inscatter1 tiles,
+// SetDesc::tground on the dJ tiles), after the T pass
+hipError_t launch_tview(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp, hipStream_t st);
+hipError_t launch_tground(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS, hipStream_t st);
 // linT / lin4: the sets' parameterisation flags (one per batch), compiled in
 hipError_t launch_inscatter1(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
                              bool linT, hipStream_t st);

@@ -60,6 +60,10 @@ __global__ __launch_bounds__(256) void k_transmittance(const SetDesc *__restrict
         o.w = 0.0f;
         o = q16(o, s.hs);
         s.T4[t] = o;
+        if (s.tau4)
+            s.tau4[t] = make_float4((float)((double)s.bR[0] * dR + (double)s.bME[0] * dM),
+                                    (float)((double)s.bR[1] * dR + (double)s.bME[1] * dM),
+                                    (float)((double)s.bR[2] * dR + (double)s.bME[2] * dM), 0.0f);
         if (s.outT) {
             s.outT[3 * t + 0] = o.x;
             s.outT[3 * t + 1] = o.y;
@@ -176,21 +180,16 @@ __global__ __launch_bounds__(256) void k_inscatter1(const SetDesc *__restrict__ 
             const double dx = limit_d(s, r, mu) / nsamp;
             if (i == 0) rowdx[row] = (float)dx;
             const bool up = mu > 0.0;
-            const float3 Tfix = T_lookup(s, (float)(r - s.Rg), (float)(up ? mu : -mu));
             S1Entry en;
             en.ti = i * dx;
             en.q = rho2 + en.ti * (en.ti + 2.0 * rmu);                 // ri^2 - Rg^2
             const float ri = sqrtf((float)(en.q + s.Rg2));
             const float h1 = (float)en.q / (ri + s.f_Rg);              // ri - Rg (< 0 at a ground hit)
             const float mu1 = (float)(rmu + en.ti) / ri;
-            float3 Tseg;
-            if (up) {
-                const float3 b = T_lookup(s, h1, mu1);
-                Tseg = make_float3(fminf(Tfix.x / b.x, 1.f), fminf(Tfix.y / b.y, 1.f), fminf(Tfix.z / b.z, 1.f));
-            } else {
-                const float3 a = T_lookup(s, h1, -mu1);
-                Tseg = make_float3(fminf(a.x / Tfix.x, 1.f), fminf(a.y / Tfix.y, 1.f), fminf(a.z / Tfix.z, 1.f));
-            }
+            // transmittance(r, mu, t_i) (common.glsl:272-282)
+            const float3 Tseg = s.tview ? ld3(gptr(s.tview) + ((size_t)l * s.RMU + yb + row) * (nsamp + 1) + i)
+                              : up      ? T_ratio_lin(s, (float)(r - s.Rg), (float)mu, h1, mu1)
+                                        : T_ratio_lin(s, h1, -mu1, (float)(r - s.Rg), (float)-mu);
             const float w = (i == 0 || i == nsamp) ? 0.5f : 1.0f;
             en.T[0] = Tseg.x * w; en.T[1] = Tseg.y * w; en.T[2] = Tseg.z * w;
             en.iri = 1.0f / ri;
@@ -404,14 +403,9 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
                 const double r1 = sqrt(r1sq);
                 const double mu1 = (s.Rg * mug + dg) / r1;
                 const float h1 = (float)((r1sq - s.Rg2) / (r1 + s.Rg));
-                float3 tr;
-                if (mug > 0.0) {
-                    float3 a = T_lookup(s, 0.0f, (float)mug), b = T_lookup(s, h1, (float)mu1);
-                    tr = make_float3(fminf(a.x / b.x, 1.f), fminf(a.y / b.y, 1.f), fminf(a.z / b.z, 1.f));
-                } else {
-                    float3 a = T_lookup(s, h1, (float)-mu1), b = T_lookup(s, 0.0f, (float)-mug);
-                    tr = make_float3(fminf(a.x / b.x, 1.f), fminf(a.y / b.y, 1.f), fminf(a.z / b.z, 1.f));
-                }
+                const float3 tr = s.tground ? ld3(gptr(s.tground) + (size_t)l * NS + it)
+                                  : mug > 0.0 ? T_ratio_lin(s, 0.0f, (float)mug, h1, (float)mu1)
+                                              : T_ratio_lin(s, h1, (float)-mu1, 0.0f, (float)-mug);
                 const float gr = (float)(s.alpha / kPI);
                 g0 = make_float4(gr * tr.x, gr * tr.y, gr * tr.z, 1.0f);
                 // E texel coordinate ((r muS + dg nu1)/Rg + 0.2)/1.2 SW - 0.5 as c1 pos + ce0
@@ -862,14 +856,9 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
             // transmittance(r, mu, ti) (common.glsl:272-282)
             const float h1 = (float)(q / (ri + s.Rg));
             const float mu1 = (float)(rmui / ri);
-            float3 Tseg;
-            if (mu > 0.0) {
-                const float3 a = T_lookup(s, (float)(r - s.Rg), (float)mu), b = T_lookup(s, h1, mu1);
-                Tseg = make_float3(fminf(a.x / b.x, 1.f), fminf(a.y / b.y, 1.f), fminf(a.z / b.z, 1.f));
-            } else {
-                const float3 a = T_lookup(s, h1, -mu1), b = T_lookup(s, (float)(r - s.Rg), (float)-mu);
-                Tseg = make_float3(fminf(a.x / b.x, 1.f), fminf(a.y / b.y, 1.f), fminf(a.z / b.z, 1.f));
-            }
+            const float3 Tseg = s.tview ? ld3(gptr(s.tview) + ((size_t)l * s.RMU + yb + row) * (nsamp + 1) + i)
+                              : mu > 0.0 ? T_ratio_lin(s, (float)(r - s.Rg), (float)mu, h1, mu1)
+                                         : T_ratio_lin(s, h1, -mu1, (float)(r - s.Rg), (float)-mu);
             const float wt = (i == 0 || i == nsamp) ? 0.5f : 1.0f;
             SnEntry en;
             en.w = make_float4((1.f - az) * (1.f - ay), (1.f - az) * ay, az * (1.f - ay), az * ay);
@@ -1020,6 +1009,71 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// Transmittance tables of one precompute (after the T pass, before the passes
+// that use them): every ratio T(x)/T(y) of common.glsl:272-282 the later passes
+// take depends only on the T table and the geometry, not on the order.
+//  k_tview:   T(r, mu, t_i) along the view ray of each (layer, mu row) at the
+//             inscatter samples t_i = i limit(r, mu) / nsamp (inscatter1.glsl,
+//             inscatterN.glsl), same tiles as k_inscatter1;
+//  k_tground: T from the ground point seen in direction theta to the layer
+//             radius (inscatterS.glsl:129-138), per (layer, theta row); run on
+//             the dJ tiles, by the workgroup of mu_s column 0 of each layer.
+// Both use T_ratio (underflow-free with the optical-depth table).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_tview(const SetDesc *__restrict__ sets,
+                                               const uint2 *__restrict__ tiles, int nsamp) {
+    const uint2 tl = tiles[blockIdx.x];
+    const SetDesc &s = sets[tl.x];
+    const int RPW = sn_rows_per_wg(s.W4, kNRows);
+    const int ngy = (s.RMU + RPW - 1) / RPW;
+    const int l = (int)tl.y / ngy;
+    const int yb = ((int)tl.y % ngy) * RPW;
+    const int nrows = min(RPW, s.RMU - yb);
+    const LayerGeom L = s.layers[l];
+    const double r = L.r;
+    const double rho2 = r * r - s.Rg2;
+    for (int e = threadIdx.x; e < nrows * (nsamp + 1); e += blockDim.x) {
+        const int row = e / (nsamp + 1), i = e % (nsamp + 1);
+        double mu, muS_u, nu_u;
+        mu_mus_nu(s, L, 0, yb + row, mu, muS_u, nu_u);
+        const double rmu = r * mu;
+        const double ti = i * (limit_d(s, r, mu) / nsamp);
+        const double q = rho2 + ti * (ti + 2.0 * rmu);   // ri^2 - Rg^2
+        const double ri = sqrt(q + s.Rg2);
+        const float h1 = (float)(q / (ri + s.Rg));
+        const float mu1 = (float)((rmu + ti) / ri);
+        const float3 t = mu > 0.0 ? T_ratio(s, (float)(r - s.Rg), (float)mu, h1, mu1)
+                                  : T_ratio(s, h1, -mu1, (float)(r - s.Rg), (float)-mu);
+        s.tview[((size_t)l * s.RMU + yb + row) * (nsamp + 1) + i] = make_float4(t.x, t.y, t.z, 0.f);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tground(const SetDesc *__restrict__ sets,
+                                                 const uint2 *__restrict__ tiles, int NS) {
+    const uint2 tl = tiles[blockIdx.x];
+    const SetDesc &s = sets[tl.x];
+    if ((int)tl.y % s.RMUS != 0) return;   // one workgroup per layer
+    const int l = (int)tl.y / s.RMUS;
+    const double r = fmin(fmax(s.layers[l].r, s.Rg), s.Rt);   // inscatterS.glsl:112
+    for (int it = threadIdx.x; it < NS; it += blockDim.x) {
+        const double ct = cos((it + 0.5) * (kPI / NS));
+        const double q = s.Rg / r;
+        float3 tr = make_float3(0.f, 0.f, 0.f);
+        if (ct < -sqrt(1.0 - q * q)) {
+            const double dg = -r * ct - sqrt(r * r * (ct * ct - 1.0) + s.Rg2);
+            const double mug = -(r * ct + dg) / s.Rg;
+            const double r1sq = s.Rg2 + dg * dg + 2.0 * s.Rg * mug * dg;
+            const double r1 = sqrt(r1sq);
+            const double mu1 = (s.Rg * mug + dg) / r1;
+            const float h1 = (float)((r1sq - s.Rg2) / (r1 + s.Rg));
+            tr = mug > 0.0 ? T_ratio(s, 0.0f, (float)mug, h1, (float)mu1)
+                           : T_ratio(s, h1, (float)-mu1, 0.0f, (float)-mug);
+        }
+        s.tground[(size_t)l * NS + it] = make_float4(tr.x, tr.y, tr.z, 0.f);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Numeric validation of finished tables: counts non-finite values, T outside
 // [0, 1] and negative E / S entries into flags[set].
 // ---------------------------------------------------------------------------
@@ -1075,6 +1129,16 @@ hipError_t launch_transmittance(const SetDesc *sets, const uint2 *tiles, int nti
                                 hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_transmittance, dim3(ntiles), dim3(256), 0, st, sets, tiles, nsamp);
+    return hipGetLastError();
+}
+hipError_t launch_tview(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp, hipStream_t st) {
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tview, dim3(ntiles), dim3(256), 0, st, sets, tiles, nsamp);
+    return hipGetLastError();
+}
+hipError_t launch_tground(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS, hipStream_t st) {
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tground, dim3(ntiles), dim3(64), 0, st, sets, tiles, NS);
     return hipGetLastError();
 }
 hipError_t launch_irradiance1(const SetDesc *sets, const uint2 *tiles, int ntiles,

@@ -302,6 +302,33 @@ def test_random_mixed_batches_match_oracle(eng, seed):
         _assert("S[%d]" % i, S.double().cpu().numpy(), oS, tol)
 
 
+def _dense(Rg, dR, dRL, HR, HM, sR, bM, g, alpha=0.1):
+    q = small(res=(6, 16, 6, 4), T=(48, 12), E=(12, 6))
+    q.Rg, q.Rt, q.RL = Rg, Rg + dR, Rg + dR + dRL
+    q.HR, q.HM, q.mieG, q.AVERAGE_GROUND_REFLECTANCE = HR, HM, g, alpha
+    q.betaR.x, q.betaR.y, q.betaR.z = 5.8e-3 * sR, 1.35e-2 * sR, 3.31e-2 * sR
+    q.betaMSca.x = q.betaMSca.y = q.betaMSca.z = bM
+    q.betaMEx.x = q.betaMEx.y = q.betaMEx.z = bM / 0.9
+    return q
+
+
+@pytest.mark.parametrize("which", ["large_planet_dense_haze", "thick_mie", "deep_rayleigh"])
+def test_transmittance_far_below_fp32_range(eng, which):
+    """Atmospheres whose grazing transmittances fall far below FP32's range
+    (e^-300 and beyond): the segment ratios T(x)/T(y) come from the
+    optical-depth table in log space, so the tables stay within the 1e-3 bar of
+    the FP64 oracle (with plain FP32 quotients these cases were off by 1e2-1e3)."""
+    p = {"large_planet_dense_haze": _dense(59594.8, 586.9, 4.6, 48.2, 1.25, 6.3, 1.6e-2, -0.3),
+         "thick_mie": _dense(51367.6, 316.8, 5.5, 0.14, 19.75, 0.164, 0.44, -0.33),
+         "deep_rayleigh": _dense(8290.8, 1.29, 2.4, 1.97, 23.9, 55.4, 0.11, 0.57)}[which]
+    T, E, S = _chain(eng, p)
+    oT, oE, oS = O.precompute(p, options())
+    assert oT[oT > 0].min() < 1e-100   # the regime this test is about
+    _assert("T", T, oT, TOL)
+    _assert("E", E, oE, TOL)
+    _assert("S", S, oS, TOL)
+
+
 def test_chain_orders_1_and_6(eng):
     p = small()
     for n in (1, 6):
