@@ -285,6 +285,7 @@ struct Plan {
     std::vector<SetDesc> hsets;
     int off[P_COUNT], cnt[P_COUNT];
     DevBuf dsets, dlayers, dtiles, ddirs, dflags, arena;
+    hipStream_t xs = nullptr;   // the engine's non-blocking stream (uploads)
     size_t dirJT_off = 0, dirJW_off = 0, dirE_off = 0;
     size_t ldsJ1 = 0, ldsJ = 0, ldsE1 = 0, ldsEN = 0;
     int chJ1 = 0, chJ = 0, chE1 = 0, chEN = 0;   // theta rows per LDS staging round
@@ -371,7 +372,15 @@ void build_dirs(int NS, int NI, std::vector<DirJHost> &jt, std::vector<float> &j
     }
 }
 
-atmo_status upload_dirs(DevBuf &buf, const atmo_options &o, size_t &offJT, size_t &offJW,
+// Host -> device copy on the engine's own non-blocking stream, waited for on
+// the host: no work on the legacy null stream, which other threads' graph
+// captures must not be made to depend on.
+hipError_t h2d(void *dst, const void *src, size_t n, hipStream_t xs) {
+    hipError_t e = hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, xs);
+    return e != hipSuccess ? e : hipStreamSynchronize(xs);
+}
+
+atmo_status upload_dirs(DevBuf &buf, hipStream_t xs, const atmo_options &o, size_t &offJT, size_t &offJW,
                         size_t &offE) {
     std::vector<DirJHost> jt;
     std::vector<float> jw;
@@ -386,7 +395,7 @@ atmo_status upload_dirs(DevBuf &buf, const atmo_options &o, size_t &offJT, size_
     memcpy(h.data() + offJT, jt.data(), jt.size() * sizeof(DirJHost));
     memcpy(h.data() + offJW, jw.data(), jw.size() * sizeof(float));
     memcpy(h.data() + offE, de.data(), de.size() * sizeof(DirEHost));
-    HIPCHK(hipMemcpy(buf.p, h.data(), total, hipMemcpyHostToDevice));
+    HIPCHK(h2d(buf.p, h.data(), total, xs));
     return ATMO_OK;
 }
 
@@ -667,7 +676,7 @@ atmo_status autotune(Plan &pl, hipStream_t st, Tuning &tu) {
     for (auto &d : hs) d.S4 = nullptr;
     DevBuf ds, dt;
     HIPCHK(ds.ensure(hs.size() * sizeof(SetDesc)));
-    HIPCHK(hipMemcpy(ds.p, hs.data(), hs.size() * sizeof(SetDesc), hipMemcpyHostToDevice));
+    HIPCHK(h2d(ds.p, hs.data(), hs.size() * sizeof(SetDesc), pl.xs));
     const int caps[3] = {1, 2, 4}, sbs[3] = {0, 1, 2};   // sb: staged samples per LDS round
     int dev = 0, maxLds = 0;
     HIPCHK(hipGetDevice(&dev));
@@ -685,7 +694,7 @@ atmo_status autotune(Plan &pl, hipStream_t st, Tuning &tu) {
             int off[P_COUNT], cnt[P_COUNT];
             build_tiles(pl.params, o, tiles, off, cnt, caps[ci], xo);
             HIPCHK(dt.ensure(tiles.size() * sizeof(uint2)));
-            HIPCHK(hipMemcpy(dt.p, tiles.data(), tiles.size() * sizeof(uint2), hipMemcpyHostToDevice));
+            HIPCHK(h2d(dt.p, tiles.data(), tiles.size() * sizeof(uint2), pl.xs));
             float ms = 0.f;
             atmo_status rc = time_launch(st, ms, [&]() {
                 return launch_inscatterN((const SetDesc *)ds.p, (const uint2 *)dt.p + off[P_SN], cnt[P_SN],
@@ -716,7 +725,7 @@ atmo_status apply_tuning(Plan &pl, const Tuning &tu) {
         std::vector<uint2> tiles;
         build_tiles(pl.params, pl.opts, tiles, pl.off, pl.cnt, pl.snCap, pl.snXcd, &pl.jseg);
         HIPCHK(pl.dtiles.ensure(tiles.size() * sizeof(uint2)));
-        HIPCHK(hipMemcpy(pl.dtiles.p, tiles.data(), tiles.size() * sizeof(uint2), hipMemcpyHostToDevice));
+        HIPCHK(h2d(pl.dtiles.p, tiles.data(), tiles.size() * sizeof(uint2), pl.xs));
     }
     pl.drop_graph();
     return ATMO_OK;
@@ -725,6 +734,7 @@ atmo_status apply_tuning(Plan &pl, const Tuning &tu) {
 atmo_status build_plan(atmo_engine *e, const atmo_params *sets, int nsets, const atmo_options &o,
                        std::unique_ptr<Plan> &out) {
     std::unique_ptr<Plan> pl(new Plan);
+    pl->xs = e->cap;
     pl->params.assign(sets, sets + nsets);
     pl->opts = o;
     pl->jG = o.texels_per_thread;   // 0: each set with its own G until the tuner has run
@@ -745,8 +755,7 @@ atmo_status build_plan(atmo_engine *e, const atmo_params *sets, int nsets, const
         for (int l = 0; l < sets[i].RES_R; ++l) layers.push_back(layer_geom(sets[i], l));
     }
     HIPCHK(pl->dlayers.ensure(layers.size() * sizeof(LayerGeom)));
-    HIPCHK(hipMemcpy(pl->dlayers.p, layers.data(), layers.size() * sizeof(LayerGeom),
-                     hipMemcpyHostToDevice));
+    HIPCHK(h2d(pl->dlayers.p, layers.data(), layers.size() * sizeof(LayerGeom), pl->xs));
     // descriptors (output pointers are filled in per call)
     pl->hsets.resize(nsets);
     char *ar = (char *)pl->arena.p;
@@ -776,9 +785,9 @@ atmo_status build_plan(atmo_engine *e, const atmo_params *sets, int nsets, const
     std::vector<uint2> tiles;
     build_tiles(pl->params, o, tiles, pl->off, pl->cnt, 4, 1, &pl->jseg);
     HIPCHK(pl->dtiles.ensure(tiles.size() * sizeof(uint2)));
-    HIPCHK(hipMemcpy(pl->dtiles.p, tiles.data(), tiles.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    HIPCHK(h2d(pl->dtiles.p, tiles.data(), tiles.size() * sizeof(uint2), pl->xs));
     HIPCHK(pl->dflags.ensure(nsets * sizeof(unsigned)));
-    atmo_status st = upload_dirs(pl->ddirs, o, pl->dirJT_off, pl->dirJW_off, pl->dirE_off);
+    atmo_status st = upload_dirs(pl->ddirs, pl->xs, o, pl->dirJT_off, pl->dirJW_off, pl->dirE_off);
     if (st != ATMO_OK) return st;
     int maxNU = 0, maxSW = 0;
     for (int i = 0; i < nsets; ++i) {
@@ -940,7 +949,7 @@ atmo_status atmo_engine_precompute(atmo_engine *e, const atmo_params *sets, int 
     }
     if (changed) {
         HIPCHK(hipStreamSynchronize(hs));
-        HIPCHK(hipMemcpy(pl.dsets.p, pl.hsets.data(), nsets * sizeof(SetDesc), hipMemcpyHostToDevice));
+        HIPCHK(h2d(pl.dsets.p, pl.hsets.data(), nsets * sizeof(SetDesc), e->cap));
     }
     if (o.flags & ATMO_FLAG_TIME_PASSES) {
         PassTimer tm;
@@ -956,20 +965,34 @@ atmo_status atmo_engine_precompute(atmo_engine *e, const atmo_params *sets, int 
     } else if (o.flags & ATMO_FLAG_NO_GRAPH) {
         if ((st = launch_all(pl, hs)) != ATMO_OK) return st;
     } else {
+        bool eager = false;
         if (!pl.gexec) {
             HIPCHK(hipStreamBeginCapture(e->cap, hipStreamCaptureModeThreadLocal));
             atmo_status cs = launch_all(pl, e->cap);
             hipGraph_t g = nullptr;
             hipError_t ce = hipStreamEndCapture(e->cap, &g);
-            if (cs != ATMO_OK) {
+            if (ce == hipErrorStreamCaptureInvalidated) {
+                // another thread's work on the legacy stream can invalidate a
+                // capture in progress: run this call's launches directly and
+                // capture again on the next call
                 if (g) (void)hipGraphDestroy(g);
-                return cs;
+                (void)hipGetLastError();
+                eager = true;
+            } else {
+                if (cs != ATMO_OK) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return cs;
+                }
+                HIPCHK(ce);
+                pl.graph = g;
+                HIPCHK(hipGraphInstantiate(&pl.gexec, g, nullptr, nullptr, 0));
             }
-            HIPCHK(ce);
-            pl.graph = g;
-            HIPCHK(hipGraphInstantiate(&pl.gexec, g, nullptr, nullptr, 0));
         }
-        HIPCHK(hipGraphLaunch(pl.gexec, hs));
+        if (eager) {
+            if ((st = launch_all(pl, hs)) != ATMO_OK) return st;
+        } else {
+            HIPCHK(hipGraphLaunch(pl.gexec, hs));
+        }
     }
     if (!pl.tuned && (o.orders >= 2) && !(o.flags & ATMO_FLAG_NO_AUTOTUNE)) {
         const std::string key = shape_key(pl.params, o);
@@ -1051,7 +1074,7 @@ static atmo_status pass_setup(atmo_engine *e, const atmo_params *p, const atmo_o
     std::vector<LayerGeom> layers;
     for (int l = 0; l < p->RES_R; ++l) layers.push_back(layer_geom(*p, l));
     HIPCHK(e->pass_layers.ensure(layers.size() * sizeof(LayerGeom)));
-    HIPCHK(hipMemcpy(e->pass_layers.p, layers.data(), layers.size() * sizeof(LayerGeom), hipMemcpyHostToDevice));
+    HIPCHK(h2d(e->pass_layers.p, layers.data(), layers.size() * sizeof(LayerGeom), e->cap));
     s.layers = (const LayerGeom *)e->pass_layers.p;
     return ATMO_OK;
 }
@@ -1061,7 +1084,7 @@ static atmo_status pass_run(atmo_engine *e, const atmo_params *p, const atmo_opt
     hipStream_t hs = (hipStream_t)stream;
     HIPCHK(hipStreamSynchronize(hs));
     HIPCHK(e->pass_sets.ensure(sizeof(SetDesc)));
-    HIPCHK(hipMemcpy(e->pass_sets.p, &s, sizeof(SetDesc), hipMemcpyHostToDevice));
+    HIPCHK(h2d(e->pass_sets.p, &s, sizeof(SetDesc), e->cap));
     std::vector<uint2> tiles;
     int off[P_COUNT], cnt[P_COUNT];
     std::vector<atmo_params> ps(1, *p);
@@ -1069,9 +1092,9 @@ static atmo_status pass_run(atmo_engine *e, const atmo_params *p, const atmo_opt
     ot.orders = std::max(ot.orders, 2);  // every pass gets its tiles
     build_tiles(ps, ot, tiles, off, cnt);
     HIPCHK(e->pass_tiles.ensure(tiles.size() * sizeof(uint2)));
-    HIPCHK(hipMemcpy(e->pass_tiles.p, tiles.data(), tiles.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    HIPCHK(h2d(e->pass_tiles.p, tiles.data(), tiles.size() * sizeof(uint2), e->cap));
     size_t oJT, oJW, oE;
-    atmo_status st = upload_dirs(e->pass_dirs, o, oJT, oJW, oE);
+    atmo_status st = upload_dirs(e->pass_dirs, e->cap, o, oJT, oJW, oE);
     if (st != ATMO_OK) return st;
     const SetDesc *S = (const SetDesc *)e->pass_sets.p;
     const uint2 *T = (const uint2 *)e->pass_tiles.p;
