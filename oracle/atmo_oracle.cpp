@@ -29,6 +29,11 @@
  *   D5 the transmittance ratios min(T(x)/T(y), 1) (common.glsl:272-282 and the
  *      consumer's) take min as GPUs execute GLSL min, IEEE minNum: a 0/0 ratio
  *      (both lookups 0: FP16 storage, or T underflowing) gives 1.
+ *   D6 in oracle_precompute (FP64 tables) the segment ratio is evaluated from
+ *      the optical depths tau = -ln T of the T texels: T(x) = e^-m sum w_i
+ *      e^(m - tau_i), m the smallest depth among the taps of nonzero weight.
+ *      Equal to the ratio of the bilinear lookups in exact arithmetic; finite
+ *      where both lookups are below FP64's range (tau > 708).
  * Uniforms are FP32 in the reference; the oracle takes the FP32 parameter
  * values and the FP32-rounded per-layer r and dhdH (PreprocessAtmo.cpp:255-271
  * computes them in double and uploads floats) and does everything else in FP64.
@@ -55,6 +60,7 @@ struct P {
     int TW, TH, SW, SH, RR, RMU, RMUS, RNU;
     int orders, tS, iS, irrS, sphS;
     bool linT, lin4;   /* linear parameterisations (common.glsl:105-106 undefined) */
+    const double *tau; /* optical depths -ln T of the T texels (D6), or null */
 };
 
 P mk(const atmo_params *a, const atmo_options *o) {
@@ -70,6 +76,7 @@ P mk(const atmo_params *a, const atmo_options *o) {
     p.RR = a->RES_R; p.RMU = a->RES_MU; p.RMUS = a->RES_MU_S; p.RNU = a->RES_NU;
     /* common.glsl:94-97 and PreprocessAtmo.cpp:351 */
     p.orders = 4; p.tS = 500; p.iS = 50; p.irrS = 32; p.sphS = 16;
+    p.tau = nullptr;
     p.linT = o && (o->flags & ATMO_FLAG_LINEAR_TRANSMITTANCE);
     p.lin4 = o && (o->flags & ATMO_FLAG_LINEAR_INSCATTER);
     if (o) {
@@ -205,8 +212,47 @@ void T_lookup(const P &p, const double *T, double r, double mu, double out[3]) {
     sample2(T, p.TW, p.TH, 3, u, v, out);
 }
 
+/* ln T(r, mu) per channel from the optical-depth table (D6): -m + ln S */
+void T_log_lookup(const P &p, double r, double mu, double m[3], double S[3]) {
+    double u, v;
+    transmittanceUV(p, r, mu, u, v);
+    int i0, i1, j0, j1;
+    double a, b;
+    axis(u, p.TW, i0, i1, a);
+    axis(v, p.TH, j0, j1, b);
+    const size_t idx[4] = {(size_t)j0 * p.TW + i0, (size_t)j0 * p.TW + i1, (size_t)j1 * p.TW + i0,
+                           (size_t)j1 * p.TW + i1};
+    const double w[4] = {(1 - a) * (1 - b), a * (1 - b), (1 - a) * b, a * b};
+    for (int c = 0; c < 3; ++c) {
+        double mc = INFINITY, Sc = 0.0;
+        for (int q = 0; q < 4; ++q)
+            if (w[q] > 0.0) mc = std::min(mc, p.tau[idx[q] * 3 + c]);
+        for (int q = 0; q < 4; ++q)
+            if (w[q] > 0.0) Sc += w[q] * std::exp(mc - p.tau[idx[q] * 3 + c]);
+        m[c] = mc;
+        S[c] = Sc;
+    }
+}
+
 /* common.glsl:272-282 transmittance(r, mu, d) between x and x + d*v */
 void T_segment(const P &p, const double *T, double r, double mu, double d, double out[3]) {
+    if (p.tau) {   /* D6 */
+        const double r1 = std::sqrt(r * r + d * d + 2.0 * r * mu * d);
+        const double mu1 = (r * mu + d) / r1;
+        double ma[3], Sa[3], mb[3], Sb[3];
+        if (mu > 0.0) {
+            T_log_lookup(p, r, mu, ma, Sa);
+            T_log_lookup(p, r1, mu1, mb, Sb);
+        } else {
+            T_log_lookup(p, r1, -mu1, ma, Sa);
+            T_log_lookup(p, r, -mu, mb, Sb);
+        }
+        for (int c = 0; c < 3; ++c) {
+            const double lr = (mb[c] - ma[c]) + std::log(Sa[c] / Sb[c]);
+            out[c] = lr >= 0.0 ? 1.0 : std::exp(lr);
+        }
+        return;
+    }
     double r1 = std::sqrt(r * r + d * d + 2.0 * r * mu * d);
     double mu1 = (r * mu + d) / r1;
     double a[3], b[3];
@@ -357,6 +403,23 @@ double opticalDepth(const P &p, double H, double r, double mu) {
     return mu < -std::sqrt(1.0 - (p.Rg / r) * (p.Rg / r)) ? 1e9 : result;
 }
 
+void set_threads();
+
+/* D6: the optical depths tau = -ln T of the T texels (the exponent of
+ * transmittance.glsl:93-98), [TH][TW][3] */
+void tau_table(const P &p, double *tau) {
+    set_threads();
+#pragma omp parallel for schedule(dynamic)
+    for (int j = 0; j < p.TH; ++j)
+        for (int i = 0; i < p.TW; ++i) {
+            double r, mu;
+            transmittanceRMu(p, i, j, r, mu);
+            const double dR = opticalDepth(p, p.HR, r, mu), dM = opticalDepth(p, p.HM, r, mu);
+            for (int c = 0; c < 3; ++c)
+                tau[((size_t)j * p.TW + i) * 3 + c] = p.bR[c] * dR + p.bME[c] * dM;
+        }
+}
+
 void set_threads() {
 #ifdef _OPENMP
     if (g_threads > 0) omp_set_num_threads(g_threads);
@@ -430,6 +493,16 @@ void oracle_transmittance_segment(const atmo_params *a, const double *T, double 
                                   double d, double out[3]) {
     T_segment(mk(a, nullptr), T, r, mu, d, out);
 }
+void oracle_transmittance_segment_tau(const atmo_params *a, const atmo_options *o,
+                                      const double *tau, double r, double mu, double d,
+                                      double out[3]) {
+    P p = mk(a, o);
+    p.tau = tau;
+    T_segment(p, nullptr, r, mu, d, out);
+}
+void oracle_optical_depth_table(const atmo_params *a, const atmo_options *o, double *tau) {
+    tau_table(mk(a, o), tau);
+}
 double oracle_optical_depth(const atmo_params *a, const atmo_options *o, double H, double r,
                             double mu) {
     return opticalDepth(mk(a, o), H, r, mu);
@@ -489,9 +562,7 @@ void inscatter1_integrand(const P &p, const double *T, double r, double mu, doub
 }  // namespace
 
 /* inscatter1.glsl:117-150, one draw per layer (PreprocessAtmo.cpp:285-297) */
-void oracle_pass_inscatter1(const atmo_params *a, const atmo_options *o, const double *T,
-                            double *dSR, double *dSM) {
-    P p = mk(a, o);
+static void pass_inscatter1(const P &p, const double *T, double *dSR, double *dSM) {
     int W = p.RMUS * p.RNU;
     set_threads();
 #pragma omp parallel for collapse(2) schedule(dynamic)
@@ -527,6 +598,11 @@ void oracle_pass_inscatter1(const atmo_params *a, const atmo_options *o, const d
     }
 }
 
+void oracle_pass_inscatter1(const atmo_params *a, const atmo_options *o, const double *T,
+                            double *dSR, double *dSM) {
+    pass_inscatter1(mk(a, o), T, dSR, dSM);
+}
+
 /* copyInscatter1.glsl:104-109 (D4) */
 void oracle_pass_copy_inscatter1(const atmo_params *a, const double *dSR, const double *dSM,
                                  double *S) {
@@ -541,10 +617,8 @@ void oracle_pass_copy_inscatter1(const atmo_params *a, const double *dSR, const 
 }
 
 /* inscatterS.glsl:111-189 */
-void oracle_pass_inscatterS(const atmo_params *a, const atmo_options *o, const double *T,
-                            const double *dE, const double *dSR, const double *dSM, int first,
-                            double *dJ) {
-    P p = mk(a, o);
+static void pass_inscatterS(const P &p, const double *T, const double *dE, const double *dSR,
+                            const double *dSM, int first, double *dJ) {
     int W = p.RMUS * p.RNU;
     const int NS = p.sphS;
     const double dphi = kPI / NS, dtheta = kPI / NS;
@@ -615,6 +689,12 @@ void oracle_pass_inscatterS(const atmo_params *a, const atmo_options *o, const d
     }
 }
 
+void oracle_pass_inscatterS(const atmo_params *a, const atmo_options *o, const double *T,
+                            const double *dE, const double *dSR, const double *dSM, int first,
+                            double *dJ) {
+    pass_inscatterS(mk(a, o), T, dE, dSR, dSM, first, dJ);
+}
+
 /* irradianceN.glsl:85-114 */
 void oracle_pass_irradianceN(const atmo_params *a, const atmo_options *o, const double *dSR,
                              const double *dSM, int first, double *dE) {
@@ -656,9 +736,7 @@ void oracle_pass_irradianceN(const atmo_params *a, const atmo_options *o, const 
 }
 
 /* inscatterN.glsl:105-131 */
-void oracle_pass_inscatterN(const atmo_params *a, const atmo_options *o, const double *T,
-                            const double *dJ, double *dSR) {
-    P p = mk(a, o);
+static void pass_inscatterN(const P &p, const double *T, const double *dJ, double *dSR) {
     int W = p.RMUS * p.RNU;
     set_threads();
 #pragma omp parallel for collapse(2) schedule(dynamic)
@@ -696,6 +774,11 @@ void oracle_pass_inscatterN(const atmo_params *a, const atmo_options *o, const d
             }
         }
     }
+}
+
+void oracle_pass_inscatterN(const atmo_params *a, const atmo_options *o, const double *T,
+                            const double *dJ, double *dSR) {
+    pass_inscatterN(mk(a, o), T, dJ, dSR);
 }
 
 /* copyInscatterN.glsl:105-110 with the ADD blend of PreprocessAtmo.cpp:336,350 (D4) */
@@ -762,9 +845,18 @@ void oracle_precompute(const atmo_params *a, const atmo_options *o, double *T, d
     std::vector<double> dE(nE * 3), dSR(n4 * 3), dSM(n4 * 3), dJ(n4 * 3);
     oracle_pass_transmittance(a, o, T);
     if (h) q16_all(T, nT);
+    /* D6: optical depths of the T texels, for the segment ratios of the chain
+     * (FP16 storage keeps the reference's quotient of stored values) */
+    std::vector<double> tau;
+    if (!h) {
+        tau.resize(nT);
+        tau_table(p, tau.data());
+    }
+    P pc = p;
+    pc.tau = h ? nullptr : tau.data();
     oracle_pass_irradiance1(a, o, T, dE.data());
     if (h) q16_all(dE.data(), dE.size());
-    oracle_pass_inscatter1(a, o, T, dSR.data(), dSM.data());
+    pass_inscatter1(pc, T, dSR.data(), dSM.data());
     if (h) {
         q16_all(dSR.data(), dSR.size());
         q16_all(dSM.data(), dSM.size());
@@ -773,11 +865,11 @@ void oracle_precompute(const atmo_params *a, const atmo_options *o, double *T, d
     oracle_pass_copy_inscatter1(a, dSR.data(), dSM.data(), S);
     for (int order = 2; order <= p.orders; ++order) {
         int first = order == 2;
-        oracle_pass_inscatterS(a, o, T, dE.data(), dSR.data(), dSM.data(), first, dJ.data());
+        pass_inscatterS(pc, T, dE.data(), dSR.data(), dSM.data(), first, dJ.data());
         if (h) q16_all(dJ.data(), dJ.size());
         oracle_pass_irradianceN(a, o, dSR.data(), dSM.data(), first, dE.data());
         if (h) q16_all(dE.data(), dE.size());
-        oracle_pass_inscatterN(a, o, T, dJ.data(), dSR.data());
+        pass_inscatterN(pc, T, dJ.data(), dSR.data());
         if (h) q16_all(dSR.data(), dSR.size());
         oracle_pass_copy_irradiance(a, 1.0, dE.data(), E);
         if (h) q16_all(E, nE * 3);

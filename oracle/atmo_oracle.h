@@ -60,6 +60,13 @@ ORACLE_API void oracle_transmittance_lookup(const atmo_params *p, const double *
                                             double mu, double out[3]);
 ORACLE_API void oracle_transmittance_segment(const atmo_params *p, const double *T, double r,
                                              double mu, double d, double out[3]);
+/* the same segment ratio from a table of optical depths -ln T, [TH][TW][3]
+ * (deviation D6: how oracle_precompute evaluates it) */
+ORACLE_API void oracle_transmittance_segment_tau(const atmo_params *p, const atmo_options *o,
+                                                 const double *tau, double r, double mu,
+                                                 double d, double out[3]);
+ORACLE_API void oracle_optical_depth_table(const atmo_params *p, const atmo_options *o,
+                                           double *tau);
 ORACLE_API double oracle_optical_depth(const atmo_params *p, const atmo_options *o, double H,
                                        double r, double mu);
 

@@ -51,6 +51,9 @@ def lib():
             "oracle_transmittance_lookup": (None, [_pp, _dp, ctypes.c_double, ctypes.c_double, _dp]),
             "oracle_transmittance_segment": (None, [_pp, _dp, ctypes.c_double, ctypes.c_double,
                                                     ctypes.c_double, _dp]),
+            "oracle_transmittance_segment_tau": (None, [_pp, _po, _dp, ctypes.c_double,
+                                                        ctypes.c_double, ctypes.c_double, _dp]),
+            "oracle_optical_depth_table": (None, [_pp, _po, _dp]),
             "oracle_optical_depth": (ctypes.c_double, [_pp, _po, ctypes.c_double, ctypes.c_double,
                                                        ctypes.c_double]),
             "oracle_pass_transmittance": (None, [_pp, _po, _dp]),
@@ -291,6 +294,20 @@ def transmittance_segment(p, T, r, mu, d):
     o = np.zeros(3)
     lib().oracle_transmittance_segment(p, _d(T), r, mu, d, _d(o))
     return o
+
+
+def optical_depth_table(p, o=None):
+    """-ln T of every transmittance texel, (TH, TW, 3); finite where T underflows"""
+    tau = np.zeros((p.TRANSMITTANCE_H, p.TRANSMITTANCE_W, 3))
+    lib().oracle_optical_depth_table(p, _opt(o), _d(tau))
+    return tau
+
+
+def transmittance_segment_tau(p, tau, r, mu, d, o=None):
+    """the segment ratio as oracle precompute evaluates it, from optical depths"""
+    out = np.zeros(3)
+    lib().oracle_transmittance_segment_tau(p, _opt(o), _d(tau), r, mu, d, _d(out))
+    return out
 
 
 def optical_depth(p, H, r, mu, o=None):

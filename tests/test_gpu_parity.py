@@ -312,15 +312,21 @@ def _dense(Rg, dR, dRL, HR, HM, sR, bM, g, alpha=0.1):
     return q
 
 
-@pytest.mark.parametrize("which", ["large_planet_dense_haze", "thick_mie", "deep_rayleigh"])
+@pytest.mark.parametrize("which", ["large_planet_dense_haze", "thick_mie", "deep_rayleigh",
+                                   "beyond_fp64_range"])
 def test_transmittance_far_below_fp32_range(eng, which):
     """Atmospheres whose grazing transmittances fall far below FP32's range
     (e^-300 and beyond): the segment ratios T(x)/T(y) come from the
     optical-depth table in log space, so the tables stay within the 1e-3 bar of
-    the FP64 oracle (with plain FP32 quotients these cases were off by 1e2-1e3)."""
+    the FP64 oracle (with plain FP32 quotients these cases were off by 1e2-1e3).
+    In the last case both lookups of some ratios fall below FP64's range too
+    (optical depths above 745): the oracle's chain also takes its ratios from
+    optical depths there, where the quotient of lookups is 0/0."""
     p = {"large_planet_dense_haze": _dense(59594.8, 586.9, 4.6, 48.2, 1.25, 6.3, 1.6e-2, -0.3),
          "thick_mie": _dense(51367.6, 316.8, 5.5, 0.14, 19.75, 0.164, 0.44, -0.33),
-         "deep_rayleigh": _dense(8290.8, 1.29, 2.4, 1.97, 23.9, 55.4, 0.11, 0.57)}[which]
+         "deep_rayleigh": _dense(8290.8, 1.29, 2.4, 1.97, 23.9, 55.4, 0.11, 0.57),
+         "beyond_fp64_range": _dense(46427.34375, 64.5234375, 0.01171875, 42.71, 13.96, 14.77,
+                                     3.165e-4, 0.72, alpha=0.39)}[which]
     T, E, S = _chain(eng, p)
     oT, oE, oS = O.precompute(p, options())
     assert oT[oT > 0].min() < 1e-100   # the regime this test is about

@@ -477,7 +477,11 @@ def test_orders_decrease_and_tables_are_physical(P):
     assert np.all(st["S"] >= 0) and np.all(st["E"] >= 0)
     assert np.all((st["T"] >= 0) & (st["T"] <= 1))
     T, E, S = O.precompute(p, options(orders=5))
-    assert np.array_equal(S, st["S"]) and np.allclose(E, st["E"], rtol=1e-15, atol=0)
+    # the chain evaluates its segment ratios from the optical depths (exp/log
+    # of the same values as the pass-by-pass quotients): equal to rounding
+    assert np.array_equal(T, st["T"])
+    assert np.allclose(S, st["S"], rtol=1e-13, atol=0)
+    assert np.allclose(E, st["E"], rtol=1e-13, atol=0)
 
 
 def test_rayleigh_only_atmosphere_has_no_mie_alpha(P):
@@ -501,3 +505,43 @@ def test_sample_count_convergence(P):
     d12 = np.abs(S2 - S1).max() / S2.max()
     assert d12 < d01 / 3
     assert np.abs(T1 - T0).max() > np.abs(T2 - T1).max()
+
+
+def test_segment_ratio_from_optical_depths_beyond_fp64_range(P):
+    """Deviation D6: the chain's segment ratio T(x)/T(y) from the optical-depth
+    table.  The quotient of two bilinear lookups is unchanged when the table is
+    scaled per channel, so the plain quotient over exp(c - tau), c the depth at
+    the start texel (the taps then lie in FP64's range), is the exact value.
+    In a dense atmosphere whose grazing depths exceed 745 the plain quotient
+    over T itself is 0, 0/0 (1 under D5) or loses digits in subnormals, while
+    the depth form stays exact."""
+    q = tiny(P, TW=256, TH=64)
+    Rg = q.Rg = 46427.34375
+    q.Rt, q.RL = Rg + 64.5234375, Rg + 64.5234375 + 0.01171875
+    q.HR, q.HM = 42.71, 13.96
+    q.betaR = Vec3f(5.8e-3 * 14.77, 1.35e-2 * 14.77, 3.31e-2 * 14.77)
+    q.betaMSca = Vec3f(3.165e-4, 3.165e-4, 3.165e-4)
+    q.betaMEx = Vec3f(3.165e-4 / 0.9, 3.165e-4 / 0.9, 3.165e-4 / 0.9)
+    tau = O.optical_depth_table(q)
+    T = O.transmittance(q)
+    assert np.allclose(T, np.exp(-tau), rtol=1e-13, atol=0)
+    beyond = 0
+    for (h, mu, d) in ((0.05, 0.0002, 5.0), (0.2, 0.0005, 20.0), (0.5, 0.0002, 20.0),
+                       (1.0, 0.0005, 5.0), (0.5, 0.001, 20.0), (0.2, -0.004, 20.0),
+                       (5.0, -0.05, 100.0), (30.0, 0.3, 20.0), (0.05, -0.3, 0.1)):
+        r = Rg + h
+        u, v = O.transmittance_uv(q, r, abs(mu))
+        i = min(int(u * q.TRANSMITTANCE_W), q.TRANSMITTANCE_W - 1)
+        j = min(int(v * q.TRANSMITTANCE_H), q.TRANSMITTANCE_H - 1)
+        scaled = np.exp(np.minimum(tau[j, i][None, None, :] - tau, 700.0))
+        exact = O.transmittance_segment(q, scaled, r, mu, d)
+        got = O.transmittance_segment_tau(q, tau, r, mu, d)
+        assert np.allclose(got, exact, rtol=1e-13, atol=0), (h, mu, d)
+        plain = O.transmittance_segment(q, T, r, mu, d)
+        for c in range(3):
+            if plain[c] != pytest.approx(exact[c], rel=1e-6, abs=0):
+                # lookups at or below FP64's subnormal range: the plain
+                # quotient is 0, 1 (0/0 under D5) or off by the lost digits
+                beyond += 1
+                assert 0.0 < got[c] < 1.0
+    assert beyond >= 4
