@@ -1018,7 +1018,8 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
 //  k_tground: T from the ground point seen in direction theta to the layer
 //             radius (inscatterS.glsl:129-138), per (layer, theta row); run on
 //             the dJ tiles, by the workgroup of mu_s column 0 of each layer.
-// Both use T_ratio (underflow-free with the optical-depth table).
+// Both use T_ratio_d (FP64 coordinates; underflow-free with the optical-depth
+// table).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_tview(const SetDesc *__restrict__ sets,
                                                const uint2 *__restrict__ tiles, int nsamp) {
@@ -1040,10 +1041,10 @@ __global__ __launch_bounds__(256) void k_tview(const SetDesc *__restrict__ sets,
         const double ti = i * (limit_d(s, r, mu) / nsamp);
         const double q = rho2 + ti * (ti + 2.0 * rmu);   // ri^2 - Rg^2
         const double ri = sqrt(q + s.Rg2);
-        const float h1 = (float)(q / (ri + s.Rg));
-        const float mu1 = (float)((rmu + ti) / ri);
-        const float3 t = mu > 0.0 ? T_ratio(s, (float)(r - s.Rg), (float)mu, h1, mu1)
-                                  : T_ratio(s, h1, -mu1, (float)(r - s.Rg), (float)-mu);
+        const double h1 = q / (ri + s.Rg);
+        const double mu1 = (rmu + ti) / ri;
+        const float3 t = mu > 0.0 ? T_ratio_d(s, r - s.Rg, mu, h1, mu1)
+                                  : T_ratio_d(s, h1, -mu1, r - s.Rg, -mu);
         s.tview[((size_t)l * s.RMU + yb + row) * (nsamp + 1) + i] = make_float4(t.x, t.y, t.z, 0.f);
     }
 }
@@ -1065,9 +1066,8 @@ __global__ __launch_bounds__(256) void k_tground(const SetDesc *__restrict__ set
             const double r1sq = s.Rg2 + dg * dg + 2.0 * s.Rg * mug * dg;
             const double r1 = sqrt(r1sq);
             const double mu1 = (s.Rg * mug + dg) / r1;
-            const float h1 = (float)((r1sq - s.Rg2) / (r1 + s.Rg));
-            tr = mug > 0.0 ? T_ratio(s, 0.0f, (float)mug, h1, (float)mu1)
-                           : T_ratio(s, h1, (float)-mu1, 0.0f, (float)-mug);
+            const double h1 = (r1sq - s.Rg2) / (r1 + s.Rg);
+            tr = mug > 0.0 ? T_ratio_d(s, 0.0, mug, h1, mu1) : T_ratio_d(s, h1, -mu1, 0.0, -mug);
         }
         s.tground[(size_t)l * NS + it] = make_float4(tr.x, tr.y, tr.z, 0.f);
     }
