@@ -175,19 +175,19 @@ __device__ __forceinline__ float3 T_ratio_lin(const SetDesc &s, float hx, float 
 
 // T_ratio_d is the form of the precompute's ratio tables (k_tview, k_tground:
 // one ratio per layer, mu row and view-ray sample, and per layer and theta
-// row, evaluated once per precompute), in FP64 from the texel coordinates on.
-// With the optical-depth table it is evaluated as
+// row, evaluated once per precompute).  Its texel coordinates and filter
+// weights are computed in FP64: along a grazing view ray the ratio divides two
+// lookups of nearly equal, large optical depth, and an FP32 error of 1e-7 in a
+// lookup's mu coordinate moves it by ~1e-3 there.  With the optical-depth
+// table the ratio is evaluated as
 //   T(x) = e^-m sum_i w_i e^(m - tau_i),  m = the smallest depth among the taps
 //   of nonzero weight,
 // which equals the bilinear lookup in exact arithmetic but cannot underflow:
 // two transmittances far below FP32's range (long grazing paths through dense
 // or large atmospheres) still give their finite ratio.  With FP16 storage
-// (which keeps the reference's stored T) it is the plain quotient.  FP64
-// coordinates matter along grazing view rays: the ratio then divides two
-// lookups of nearly equal, large optical depth, and an FP32 error of 1e-7 in
-// a lookup's mu coordinate moves it by ~1e-3.
+// (which keeps the reference's stored T) it is the plain quotient.
 __device__ __forceinline__ void T_taps_d(const SetDesc &s, double h, double mu, int idx[4],
-                                         double w[4]) {
+                                         float w[4]) {
     const double hc = fmax(h, 0.0), dRt = s.Rt - s.Rg;
     const double ur = s.linT ? hc / dRt : sqrt(hc / dRt);
     const double um = s.linT ? (mu + 0.15) / 1.15 : atan((mu + 0.15) / 1.15 * tan(1.5)) / 1.5;
@@ -197,47 +197,54 @@ __device__ __forceinline__ void T_taps_d(const SetDesc &s, double h, double mu, 
     gl_axis_dd(ur * s.TH - 0.5, s.TH, j0, j1, b);
     idx[0] = j0 * s.TW + i0; idx[1] = j0 * s.TW + i1;
     idx[2] = j1 * s.TW + i0; idx[3] = j1 * s.TW + i1;
-    w[0] = (1.0 - a) * (1.0 - b); w[1] = a * (1.0 - b);
-    w[2] = (1.0 - a) * b; w[3] = a * b;
+    w[0] = (float)((1.0 - a) * (1.0 - b)); w[1] = (float)(a * (1.0 - b));
+    w[2] = (float)((1.0 - a) * b); w[3] = (float)(a * b);
+}
+// ln T = -m + ln S of one channel c of a lookup with taps idx, weights w
+__device__ __forceinline__ void T_log_channel(const float *tau, const int idx[4], const float w[4],
+                                              int c, float &m, float &S) {
+    float t[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) t[q] = tau[4 * idx[q] + c];
+    m = __builtin_huge_valf();
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (w[q] > 0.0f) m = fminf(m, t[q]);
+    S = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (w[q] > 0.0f) S = fmaf(w[q], expf(m - t[q]), S);
 }
 __device__ __forceinline__ float3 T_ratio_d(const SetDesc &s, double hx, double mux, double hy,
                                             double muy) {
     int ix[4], iy[4];
-    double wx[4], wy[4], r[3];
+    float wx[4], wy[4], r[3];
     T_taps_d(s, hx, mux, ix, wx);
     T_taps_d(s, hy, muy, iy, wy);
     if (!s.tau4 || s.hs) {   // the plain quotient of the stored T (FP16 storage)
         const float *T = (const float *)s.T4;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            double a = 0.0, b = 0.0;
+            float a = 0.0f, b = 0.0f;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                a = fma(wx[q], (double)T[4 * ix[q] + c], a);
-                b = fma(wy[q], (double)T[4 * iy[q] + c], b);
+                a = fmaf(wx[q], T[4 * ix[q] + c], a);
+                b = fmaf(wy[q], T[4 * iy[q] + c], b);
             }
-            r[c] = fmin(a / b, 1.0);
+            r[c] = fminf(a / b, 1.0f);
         }
     } else {
         const float *tau = (const float *)s.tau4;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            double mx = __builtin_huge_val(), my = __builtin_huge_val(), Sx = 0.0, Sy = 0.0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (wx[q] > 0.0) mx = fmin(mx, (double)tau[4 * ix[q] + c]);
-                if (wy[q] > 0.0) my = fmin(my, (double)tau[4 * iy[q] + c]);
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (wx[q] > 0.0) Sx = fma(wx[q], exp(mx - (double)tau[4 * ix[q] + c]), Sx);
-                if (wy[q] > 0.0) Sy = fma(wy[q], exp(my - (double)tau[4 * iy[q] + c]), Sy);
-            }
-            const double lr = (my - mx) + log(Sx / Sy);
-            r[c] = lr >= 0.0 ? 1.0 : exp(lr);
+            float mx, Sx, my, Sy;
+            T_log_channel(tau, ix, wx, c, mx, Sx);
+            T_log_channel(tau, iy, wy, c, my, Sy);
+            const float lr = (my - mx) + logf(Sx / Sy);
+            r[c] = lr >= 0.0f ? 1.0f : expf(lr);
         }
     }
-    return make_float3((float)r[0], (float)r[1], (float)r[2]);
+    return make_float3(r[0], r[1], r[2]);
 }
 
 // Phase functions, common.glsl:290-297.
