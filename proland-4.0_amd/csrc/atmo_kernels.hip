@@ -753,8 +753,23 @@ struct SnEntry {
     float4 w;     // bilinear (r, mu) weights of the 4 table rows
     int4 base;    // offsets of the 4 rows (z0y0, z0y1, z1y0, z1y1)
     float4 tw;    // T(r, mu, t) * trapezoid weight (rgb), t
-    float4 ir;    // 1 / r_i
+    float4 ir;    // 1 / r_i, 0.2 (r_i - r)
 };
+
+// Texel offset of the sample's mu_s_i = (nu t + mu_s r) / r_i inside a nu
+// slice (texture4D's uMuS * RMUS - 0.5).  muSr is mu_s r per texel, or with
+// the linear parameterisation (mu_s + 0.2) r: its offset is proportional to
+// mu_s_i + 0.2 = (nu t + (mu_s + 0.2) r + 0.2 (r_i - r)) / r_i, evaluated in
+// that form because for the texels of the mu_s = -0.2 column it is a small
+// remainder (1e-5 along a short ray) that an FP32 mu_s_i, accurate to ~2e-8,
+// followed by + 0.2 would give only to a few 1e-3.
+template <bool LIN4>
+__device__ __forceinline__ float sn_txs(const SetDesc &s, float nu, float muSr, const SnEntry &en) {
+    if (LIN4)
+        return fmaxf(fmaf(nu, en.tw.w, muSr + en.ir.y) * en.ir.x, 0.0f) * (1.0f / 1.2f) *
+               (float)(s.RMUS - 1);
+    return tex4d_txs_f<false>(s, fmaf(nu, en.tw.w, muSr) * en.ir.x);
+}
 
 // STAGED = false: each texel gathers its 8 taps per sample from dJ directly
 // (global loads).  STAGED = true: per sample the workgroup first blends the
@@ -808,7 +823,8 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
             fk[j] = (float)(pos - kfl);
             k0[j] = (int)kfl;
             nuf[j] = (float)nu;
-            muSr[j] = (float)(muS * r);
+            // LIN4: (mu_s + 0.2) r, the numerator of the sample's mu_s_i + 0.2 (below)
+            muSr[j] = (float)(LIN4 ? (muS + 0.2) * r : muS * r);
             rowi[j] = row;
             // the row's trapezoid step, as the sample staging computes it (mu depends on the row only)
             dxj[j] = (float)(limit_d(s, r, mu) / nsamp);
@@ -817,7 +833,11 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
 
     typedef __attribute__((address_space(1))) const char gchar;
     gchar *J = (gchar *)s.dJ4;
-    const int W4m1 = s.W4 - 1;
+    // A lookup's texel column is kbase + txs: kbase the first column of its nu
+    // slice, txs its offset, kept apart so that the fraction ax has full FP32
+    // precision.  txs runs past RMU_S - 1 into the next slice when mu_s_i > 1
+    // (as GL's coordinate does); only the row's last column clamps (CLAMP_TO_EDGE),
+    // and i0 = kbase + is <= W4 - 2 keeps both taps in the row.
     const unsigned W4m2 = (unsigned)(s.W4 - 2);
     const unsigned jb = (unsigned)s.dJb;   // bytes per dJ texel
     for (int c0 = 0; c0 <= nsamp; c0 += kNChunk) {
@@ -865,7 +885,7 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
             en.base = make_int4((z0 * HW + y0 * s.W4) * s.dJb, (z0 * HW + y1 * s.W4) * s.dJb,
                                 (z1 * HW + y0 * s.W4) * s.dJb, (z1 * HW + y1 * s.W4) * s.dJb);
             en.tw = make_float4(Tseg.x * wt, Tseg.y * wt, Tseg.z * wt, (float)ti);
-            en.ir = make_float4((float)(1.0 / ri), 0.f, 0.f, 0.f);
+            en.ir = make_float4((float)(1.0 / ri), (float)(0.2 * (ri - r)), 0.f, 0.f);
             ent[row][js] = en;
         }
         __syncthreads();
@@ -876,15 +896,16 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
                 const int e = threadIdx.x + j * (int)blockDim.x;
                 if (e >= per) break;
                 const SnEntry *er = ent[rowi[j]];
-                const float kbase = (float)(k0[j] * s.RMUS);
+                const unsigned kbase = (unsigned)(k0[j] * s.RMUS);
+                const float txmax = (float)(W4m2 + 1 - kbase);
+                const unsigned ismax = W4m2 - kbase;
                 float a0 = acc[j][0], a1 = acc[j][1], a2 = acc[j][2];
                 for (int js = 0; js < nc; ++js) {
                     const SnEntry en = er[js];
-                    const float muSi = fmaf(nuf[j], en.tw.w, muSr[j]) * en.ir.x;
-                    // x: texel column; clamped to [0, W4-1] and split so that i0 + 1 <= W4 - 1
-                    float tx = __builtin_amdgcn_fmed3f(kbase + tex4d_txs_f<LIN4>(s, muSi), 0.0f, (float)W4m1);
-                    const unsigned i0 = min(__float2uint_rz(tx), W4m2);
-                    const float ax = tx - (float)i0;
+                    const float txs = __builtin_amdgcn_fmed3f(sn_txs<LIN4>(s, nuf[j], muSr[j], en), 0.0f, txmax);
+                    const unsigned is = min(__float2uint_rz(txs), ismax);
+                    const float ax = txs - (float)is;
+                    const unsigned i0 = kbase + is;
                     float3 v = make_float3(0.f, 0.f, 0.f);
                     const float wr[4] = {en.w.x, en.w.y, en.w.z, en.w.w};
                     const unsigned br[4] = {(unsigned)en.base.x, (unsigned)en.base.y, (unsigned)en.base.z,
@@ -899,10 +920,10 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
                         v.y = fmaf(wq, c0v.y, fmaf(wa, c1v.y - c0v.y, v.y));
                         v.z = fmaf(wq, c0v.z, fmaf(wa, c1v.z - c0v.z, v.z));
                     }
-                    if (fk[j] != 0.0f) {  // off-grid nu: blend with the next slice
-                        const float tx2 = __builtin_amdgcn_fmed3f(tx + (float)s.RMUS, 0.0f, (float)W4m1);
-                        const unsigned j0 = min(__float2uint_rz(tx2), W4m2);
-                        const float bx = tx2 - (float)j0;
+                    if (fk[j] != 0.0f) {  // off-grid nu: blend with the next slice (k0 + 1 <= RNU - 1)
+                        unsigned j0 = i0 + (unsigned)s.RMUS;
+                        float bx = ax;
+                        if (j0 > W4m2) { j0 = W4m2; bx = 1.0f; }   // past the row's last column
                         float3 v2 = make_float3(0.f, 0.f, 0.f);
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
@@ -954,23 +975,25 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
                     const int e = threadIdx.x + j * (int)blockDim.x;
                     if (e >= per) break;
                     const SnEntry *er = ent[rowi[j]];
-                    const float kbase = (float)(k0[j] * s.RMUS);
+                    const unsigned kbase = (unsigned)(k0[j] * s.RMUS);
+                    const float txmax = (float)(W4m2 + 1 - kbase);
+                    const unsigned ismax = W4m2 - kbase;
                     const int rowbase = rowi[j] * s.W4;
                     float a0 = acc[j][0], a1 = acc[j][1], a2 = acc[j][2];
                     for (int k = 0; k < ns; ++k) {
                         const SnEntry &en = er[s0 + k];
-                        const float muSi = fmaf(nuf[j], en.tw.w, muSr[j]) * en.ir.x;
-                        float tx = __builtin_amdgcn_fmed3f(kbase + tex4d_txs_f<LIN4>(s, muSi), 0.0f, (float)W4m1);
-                        const unsigned i0 = min(__float2uint_rz(tx), W4m2);
-                        const float ax = tx - (float)i0;
+                        const float txs = __builtin_amdgcn_fmed3f(sn_txs<LIN4>(s, nuf[j], muSr[j], en), 0.0f, txmax);
+                        const unsigned is = min(__float2uint_rz(txs), ismax);
+                        const float ax = txs - (float)is;
+                        const unsigned i0 = kbase + is;
                         const float4 *B = blend + (size_t)k * per + rowbase;
                         const float4 p0 = B[i0], p1 = B[i0 + 1];
                         float3 v = make_float3(fmaf(ax, p1.x - p0.x, p0.x), fmaf(ax, p1.y - p0.y, p0.y),
                                                fmaf(ax, p1.z - p0.z, p0.z));
-                        if (fk[j] != 0.0f) {  // off-grid nu: blend with the next slice
-                            const float tx2 = __builtin_amdgcn_fmed3f(tx + (float)s.RMUS, 0.0f, (float)W4m1);
-                            const unsigned j0 = min(__float2uint_rz(tx2), W4m2);
-                            const float bx = tx2 - (float)j0;
+                        if (fk[j] != 0.0f) {  // off-grid nu: blend with the next slice (k0 + 1 <= RNU - 1)
+                            unsigned j0 = i0 + (unsigned)s.RMUS;
+                            float bx = ax;
+                            if (j0 > W4m2) { j0 = W4m2; bx = 1.0f; }   // past the row's last column
                             const float4 q0 = B[j0], q1 = B[j0 + 1];
                             const float3 v2 = make_float3(fmaf(bx, q1.x - q0.x, q0.x), fmaf(bx, q1.y - q0.y, q0.y),
                                                           fmaf(bx, q1.z - q0.z, q0.z));

@@ -396,6 +396,23 @@ def test_wide_atmosphere_distribution_in_one_batch(eng):
     _assert("S1[2]", np.where(nan, 0.0, g), np.where(nan, 0.0, oS), 2e-4)
 
 
+def test_wide_atmosphere_distribution_linear_parameterisations(eng):
+    """The 24 sets of the wide distribution with both linear parameterisations
+    (common.glsl:105-106 undefined), in one batch: every table of every set
+    within 2e-4 of the oracle.  (The linear getMuMuSNu takes mu from the row
+    index, :162-166, so the set whose top layer is NaN above is finite here.)"""
+    rng = np.random.default_rng(77)
+    sets = [_wide_set(rng) for _ in range(24)]
+    o = options(flags=FLAG_LINEAR_TRANSMITTANCE | FLAG_LINEAR_INSCATTER)
+    outs = eng.precompute(sets, o)
+    torch.cuda.synchronize()
+    for i, (p, (T, E, S)) in enumerate(zip(sets, outs)):
+        oT, oE, oS = O.precompute(p, o)
+        _assert("T[%d]" % i, T.double().cpu().numpy(), oT, 2e-4)
+        _assert("E[%d]" % i, E.double().cpu().numpy(), oE, 2e-4)
+        _assert("S[%d]" % i, S.double().cpu().numpy(), oS, 2e-4)
+
+
 def _wider_set(rng):
     """A still wider distribution: Rg 100-1e5, thickness 0.3-2000, RL - Rt
     1e-3-300, H_R 0.05-200, H_M 0.05-100, beta_R 1e-3-300x Earth, grey
