@@ -753,22 +753,40 @@ struct SnEntry {
     float4 w;     // bilinear (r, mu) weights of the 4 table rows
     int4 base;    // offsets of the 4 rows (z0y0, z0y1, z1y0, z1y1)
     float4 tw;    // T(r, mu, t) * trapezoid weight (rgb), t
-    float4 ir;    // 1 / r_i, 0.2 (r_i - r)
+    float4 ir;    // 1 / r_i, r_i - r
 };
 
 // Texel offset of the sample's mu_s_i = (nu t + mu_s r) / r_i inside a nu
-// slice (texture4D's uMuS * RMUS - 0.5).  muSr is mu_s r per texel, or with
-// the linear parameterisation (mu_s + 0.2) r: its offset is proportional to
-// mu_s_i + 0.2 = (nu t + (mu_s + 0.2) r + 0.2 (r_i - r)) / r_i, evaluated in
-// that form because for the texels of the mu_s = -0.2 column it is a small
-// remainder (1e-5 along a short ray) that an FP32 mu_s_i, accurate to ~2e-8,
-// followed by + 0.2 would give only to a few 1e-3.
+// slice (texture4D's uMuS * RMUS - 0.5), evaluated relative to the texel's own
+// mu_s.  Near the start of a long, nearly horizontal ray mu_s_i moves away from
+// mu_s by 1e-5 and less per sample, and where a thin layer's dJ rises steeply
+// out of the shadow between two mu_s columns the lookup is proportional to
+// that move: an FP32 mu_s_i (~2e-8 absolute) put through the coordinate
+// formula (whose + 0.74, resp. + 0.2, cancels at the low end) would give it
+// only to a few 1e-3.  So:
+//   linear (:165), mus = (mu_s + 0.2) r per texel: offset proportional to
+//     mu_s_i + 0.2 = (nu t + (mu_s + 0.2) r + 0.2 (r_i - r)) / r_i;
+//   non-linear (:161), mus = mu_s and txb = the texel's own offset (FP64): with
+//     d = max(mu_s_i, -0.1975) - mu_b, mu_b = max(mu_s, -0.1975), and
+//     mu_s_i - mu_s = (nu t - mu_s (r_i - r)) / r_i,
+//     offset = txb + C atan(k d / (1 + k mu_b k (mu_b + d))), k = tan(1.386),
+//     C = 0.5 (RMUS - 1) / 1.1 (atan a - atan b = atan((a - b) / (1 + a b)),
+//     used while |k d| < 0.5, where 1 + a b > 0; larger moves take the
+//     reference's form, which has no cancellation there).
 template <bool LIN4>
-__device__ __forceinline__ float sn_txs(const SetDesc &s, float nu, float muSr, const SnEntry &en) {
+__device__ __forceinline__ float sn_txs(const SetDesc &s, float nu, float mus, float txb,
+                                        const SnEntry &en) {
+    const float half = 0.5f * (float)(s.RMUS - 1);
     if (LIN4)
-        return fmaxf(fmaf(nu, en.tw.w, muSr + en.ir.y) * en.ir.x, 0.0f) * (1.0f / 1.2f) *
-               (float)(s.RMUS - 1);
-    return tex4d_txs_f<false>(s, fmaf(nu, en.tw.w, muSr) * en.ir.x);
+        return fmaxf(fmaf(nu, en.tw.w, fmaf(0.2f, en.ir.y, mus)) * en.ir.x, 0.0f) * (1.0f / 1.2f) *
+               (2.0f * half);
+    const float mub = fmaxf(mus, -0.1975f);
+    const float dmu = fmaf(nu, en.tw.w, -mus * en.ir.y) * en.ir.x;   // mu_s_i - mu_s
+    const float d = fmaxf(dmu + (mus - mub), -0.1975f - mub);
+    const float k = s.f_tMuS, kb = k * mub, kd = k * d;
+    const bool near = fabsf(kd) < 0.5f;
+    const float arg = near ? kd * __builtin_amdgcn_rcpf(fmaf(kb, kb + kd, 1.0f)) : kb + kd;
+    return fmaf(atanf(arg), half * (1.0f / 1.1f), near ? txb : 0.74f * half);
 }
 
 // STAGED = false: each texel gathers its 8 taps per sample from dJ directly
@@ -780,7 +798,7 @@ __device__ __forceinline__ float sn_txs(const SetDesc &s, float nu, float muSr, 
 // they differ only in FP32 rounding order.  `sb` samples are blended per LDS
 // round (dynamic LDS: sb * rows * RNU*RMU_S float4).
 template <bool STAGED, bool LIN4>   // LIN4: linear inscatter parameterisation (per-sample mu_s)
-__global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ sets,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_inscatterN(const SetDesc *__restrict__ sets,
                                                     const uint2 *__restrict__ tiles,
                                                     int nsamp, int rpw_cap, int sb) {
     __shared__ SnEntry ent[kNRows][kNChunk];   // exactly 16 KiB: two workgroups of 64 KiB
@@ -803,13 +821,13 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
     const int nmine = (per + (int)blockDim.x - 1) / (int)blockDim.x;
     constexpr int MAXT = 4;   // W4 <= 1024 (validated on the host)
     float acc[MAXT][3];
-    float nuf[MAXT], muSr[MAXT], fk[MAXT], dxj[MAXT];
-    int k0[MAXT], rowi[MAXT];
+    float nuf[MAXT], muSr[MAXT], txb[MAXT], fk[MAXT];
+    int kq[MAXT];   // the texel's nu slice k0 (< RNU) | its row in the workgroup << 16
 #pragma unroll
     for (int j = 0; j < MAXT; ++j) {
         acc[j][0] = acc[j][1] = acc[j][2] = 0.f;
-        nuf[j] = muSr[j] = fk[j] = dxj[j] = 0.f;
-        k0[j] = rowi[j] = 0;
+        nuf[j] = muSr[j] = txb[j] = fk[j] = 0.f;
+        kq[j] = 0;
         const int e = threadIdx.x + j * (int)blockDim.x;
         if (j < nmine && e < per) {
             const int row = e / s.W4, x = e % s.W4;
@@ -821,13 +839,11 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
             if (fabs(pos - kr) < 1e-9) pos = kr;
             const double kfl = floor(pos);
             fk[j] = (float)(pos - kfl);
-            k0[j] = (int)kfl;
+            kq[j] = (int)kfl | (row << 16);
             nuf[j] = (float)nu;
-            // LIN4: (mu_s + 0.2) r, the numerator of the sample's mu_s_i + 0.2 (below)
-            muSr[j] = (float)(LIN4 ? (muS + 0.2) * r : muS * r);
-            rowi[j] = row;
-            // the row's trapezoid step, as the sample staging computes it (mu depends on the row only)
-            dxj[j] = (float)(limit_d(s, r, mu) / nsamp);
+            // the texel's terms of its samples' mu_s offsets (sn_txs)
+            muSr[j] = (float)(LIN4 ? (muS + 0.2) * r : muS);
+            txb[j] = LIN4 ? 0.0f : (float)(tex4d_umus(s, muS) * s.RMUS - 0.5);
         }
     }
 
@@ -885,7 +901,7 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
             en.base = make_int4((z0 * HW + y0 * s.W4) * s.dJb, (z0 * HW + y1 * s.W4) * s.dJb,
                                 (z1 * HW + y0 * s.W4) * s.dJb, (z1 * HW + y1 * s.W4) * s.dJb);
             en.tw = make_float4(Tseg.x * wt, Tseg.y * wt, Tseg.z * wt, (float)ti);
-            en.ir = make_float4((float)(1.0 / ri), (float)(0.2 * (ri - r)), 0.f, 0.f);
+            en.ir = make_float4((float)(1.0 / ri), (float)(ri - r), 0.f, 0.f);
             ent[row][js] = en;
         }
         __syncthreads();
@@ -895,14 +911,14 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
                 if (j >= nmine) break;
                 const int e = threadIdx.x + j * (int)blockDim.x;
                 if (e >= per) break;
-                const SnEntry *er = ent[rowi[j]];
-                const unsigned kbase = (unsigned)(k0[j] * s.RMUS);
+                const SnEntry *er = ent[kq[j] >> 16];
+                const unsigned kbase = (unsigned)((kq[j] & 0xffff) * s.RMUS);
                 const float txmax = (float)(W4m2 + 1 - kbase);
                 const unsigned ismax = W4m2 - kbase;
                 float a0 = acc[j][0], a1 = acc[j][1], a2 = acc[j][2];
                 for (int js = 0; js < nc; ++js) {
                     const SnEntry en = er[js];
-                    const float txs = __builtin_amdgcn_fmed3f(sn_txs<LIN4>(s, nuf[j], muSr[j], en), 0.0f, txmax);
+                    const float txs = __builtin_amdgcn_fmed3f(sn_txs<LIN4>(s, nuf[j], muSr[j], txb[j], en), 0.0f, txmax);
                     const unsigned is = min(__float2uint_rz(txs), ismax);
                     const float ax = txs - (float)is;
                     const unsigned i0 = kbase + is;
@@ -953,8 +969,9 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
                         if (j >= nmine) break;
                         const int e = threadIdx.x + j * (int)blockDim.x;
                         if (e >= per) break;
-                        const SnEntry &en = ent[rowi[j]][s0 + k];
-                        const unsigned xo = (unsigned)(e - rowi[j] * s.W4) * jb;
+                        const int rowj = kq[j] >> 16;
+                        const SnEntry &en = ent[rowj][s0 + k];
+                        const unsigned xo = (unsigned)(e - rowj * s.W4) * jb;
                         const float3 t0 = ld3((const gfloat4 *)(J + (unsigned)en.base.x + xo));
                         const float3 t1 = ld3((const gfloat4 *)(J + (unsigned)en.base.y + xo));
                         const float3 t2 = ld3((const gfloat4 *)(J + (unsigned)en.base.z + xo));
@@ -974,15 +991,15 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
                     if (j >= nmine) break;
                     const int e = threadIdx.x + j * (int)blockDim.x;
                     if (e >= per) break;
-                    const SnEntry *er = ent[rowi[j]];
-                    const unsigned kbase = (unsigned)(k0[j] * s.RMUS);
+                    const SnEntry *er = ent[kq[j] >> 16];
+                    const unsigned kbase = (unsigned)((kq[j] & 0xffff) * s.RMUS);
                     const float txmax = (float)(W4m2 + 1 - kbase);
                     const unsigned ismax = W4m2 - kbase;
-                    const int rowbase = rowi[j] * s.W4;
+                    const int rowbase = (kq[j] >> 16) * s.W4;
                     float a0 = acc[j][0], a1 = acc[j][1], a2 = acc[j][2];
                     for (int k = 0; k < ns; ++k) {
                         const SnEntry &en = er[s0 + k];
-                        const float txs = __builtin_amdgcn_fmed3f(sn_txs<LIN4>(s, nuf[j], muSr[j], en), 0.0f, txmax);
+                        const float txs = __builtin_amdgcn_fmed3f(sn_txs<LIN4>(s, nuf[j], muSr[j], txb[j], en), 0.0f, txmax);
                         const unsigned is = min(__float2uint_rz(txs), ismax);
                         const float ax = txs - (float)is;
                         const unsigned i0 = kbase + is;
@@ -1016,7 +1033,10 @@ __global__ __launch_bounds__(256) void k_inscatterN(const SetDesc *__restrict__ 
         if (e >= per) break;
         const int row = e / s.W4, x = e % s.W4;
         const size_t t = ((size_t)l * s.RMU + yb + row) * s.W4 + x;
-        const float dx = dxj[j];
+        // the row's trapezoid step, as the sample staging computes it (mu depends on the row only)
+        double mu, muS_u, nu_u;
+        mu_mus_nu(s, L, 0, yb + row, mu, muS_u, nu_u);
+        const float dx = (float)(limit_d(s, r, mu) / nsamp);
         const float4 o = q16(make_float4(acc[j][0] * dx, acc[j][1] * dx, acc[j][2] * dx, 0.f), s.hs);
         s.dSR4[t] = o;
         if (s.S4) {
