@@ -207,6 +207,13 @@ ATMO_API atmo_status atmo_engine_workspace_size(const atmo_params *sets, int nse
 /* ---------------------------------------------------------------------- */
 /* Pass-level entry points (one reference shader program each)             */
 /* All pointers: device float4 tables on the engine's device.              */
+/* The segment transmittances T(x)/T(y) (common.glsl:272-282) are the      */
+/* plain FP32 quotient of two lookups in the caller's T table, as in the   */
+/* reference shaders.  Where both lookups fall below FP32's range (grazing */
+/* paths with optical depths above ~87: dense or very large atmospheres)   */
+/* that quotient is 0/0 and the pass differs from an FP64 evaluation;      */
+/* atmo_engine_precompute evaluates these ratios from optical depths and   */
+/* has no such limit.                                                      */
 /* ---------------------------------------------------------------------- */
 
 /* transmittance.glsl:78-98 -> T[TRANSMITTANCE_H][TRANSMITTANCE_W] */
