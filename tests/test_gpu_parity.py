@@ -16,8 +16,8 @@ torch = pytest.importorskip("torch")
 from oracle import oracle as O  # noqa: E402
 from prolandatmo import lib as L  # noqa: E402
 from prolandatmo.metrics import rel_err, summary  # noqa: E402
-from prolandatmo.params import (AtmoParams, FLAG_HALF_STORAGE, FLAG_LINEAR_INSCATTER,  # noqa: E402
-                                FLAG_LINEAR_TRANSMITTANCE, SEED_BATCH, draw_sets, earth, options,
+from prolandatmo.params import (AtmoParams, FLAG_CHECK_NUMERICS, FLAG_HALF_STORAGE,  # noqa: E402
+                                FLAG_LINEAR_INSCATTER, FLAG_LINEAR_TRANSMITTANCE, SEED_BATCH, draw_sets, earth, options,
                                 with_res)
 
 pytestmark = pytest.mark.gpu
@@ -510,6 +510,39 @@ def test_smallest_tables_the_abi_accepts(eng):
     _assert("T", T, oT, TOL)
     _assert("E", E, oE, TOL)
     _assert("S", S, oS, TOL)
+
+
+def test_largest_tables_the_abi_accepts(eng):
+    """RES_R x RES_MU x RES_MU_S x RES_NU = 128 x 1024 x 32 x 32 = 2^27 texels,
+    the largest 4D table the ABI accepts (RES_MU_S * RES_NU at its limit of
+    1024 too), T 512x128, E 128x32.  Too large for the oracle, so:
+    * its corner texels (layers 0 and R-1, rows 0, MU/2-1, MU/2, MU-1, mu_s
+      columns 0 and M-1, nu slices 0 and N-1) have the (r, mu, mu_s, nu) of the
+      2x4x2x2 table's texels, so their single scattering equals the oracle's
+      at that size (1e-4);
+    * with 4 orders every table passes the numeric check (finite, T in [0, 1],
+      E and S >= 0), T is bit-equal to T of a small run with the same
+      transmittance size (the T pass does not depend on the 4D table), and a
+      second run gives the same bits."""
+    res = (128, 1024, 32, 32)
+    p = small(res=res, T=(512, 128), E=(128, 32))
+    q = small(res=(2, 4, 2, 2), T=(512, 128), E=(128, 32))
+    (T1, E1, S1), = eng.precompute(p, options(orders=1, flags=FLAG_CHECK_NUMERICS))
+    torch.cuda.synchronize()
+    R, MU, M, N = res
+    corners = S1[[0, R - 1]][:, [0, MU // 2 - 1, MU // 2, MU - 1]]
+    corners = corners.reshape(2, 4, N, M, 4)[:, :, [0, N - 1]][:, :, :, [0, M - 1]]
+    oS1 = O.precompute(q, options(orders=1))[2]
+    _assert("S1 corners", corners.reshape(2, 4, 4, 4).double().cpu().numpy(), oS1, 1e-4)
+    del S1, corners
+    (T, E, S), = eng.precompute(p, options(flags=FLAG_CHECK_NUMERICS))
+    torch.cuda.synchronize()
+    (Tq, _, _), = eng.precompute(q, options())
+    assert torch.equal(T, Tq)
+    S0 = S.clone()
+    eng.precompute(p, options(), outs=[(T, E, S)])
+    torch.cuda.synchronize()
+    assert torch.equal(S, S0)
 
 
 def test_golden_tiny_fixtures(eng):
