@@ -145,3 +145,24 @@ def test_consumer_validates_arguments_before_any_device_call():
     assert lib.atmo_in_scattering(p, None, None, v0, None, 0, None, None, None) == 1
     vc = C.view([0, 0, 0], [0, 0, 1])
     assert lib.atmo_sky_radiance(p, None, None, vc, None, 0, None, None, None) == 1
+
+
+def test_empty_and_null_batches_are_rejected():
+    """An empty batch (nsets 0 or negative) or missing pointers is an
+    ATMO_ERR_INVALID_ARGUMENT status, before any device work; so is a batch
+    whose second set is invalid (nothing of the first is computed either)."""
+    lib = L.lib()
+    one = L.params_array([earth()])
+    n = ctypes.c_size_t()
+    o = AtmoOptions()
+    lib.atmo_options_default(o)
+    assert lib.atmo_engine_workspace_size(one, 0, ctypes.byref(o), ctypes.byref(n)) == 1
+    assert lib.atmo_engine_workspace_size(one, -3, ctypes.byref(o), ctypes.byref(n)) == 1
+    assert lib.atmo_engine_workspace_size(None, 1, ctypes.byref(o), ctypes.byref(n)) == 1
+    assert lib.atmo_engine_workspace_size(one, 1, ctypes.byref(o), None) == 1
+    assert lib.atmo_engine_precompute(None, one, 1, ctypes.byref(o), None, None) == 1
+    bad = earth()
+    bad.RES_MU_S = 1
+    with pytest.raises(L.AtmoError) as ei:
+        L.workspace_size([earth(), bad])
+    assert ei.value.status == 1
