@@ -1,7 +1,8 @@
 /*
  * atmo_oracle.cpp — FP64 CPU oracle of Proland's atmo preprocess.
  *
- * TEST INFRASTRUCTURE ONLY (the correctness judge; never part of the product).
+ * TEST INFRASTRUCTURE ONLY (the reference results the tests compare against;
+ * never part of the product).
  *
  * A double-precision restatement, function by function, of
  *   src/atmo/sources/proland/preprocess/atmo/common.glsl        (shared math)

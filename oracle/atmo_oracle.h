@@ -1,9 +1,9 @@
 /*
  * atmo_oracle.h — FP64 CPU oracle of Proland's atmo preprocess.
  *
- * TEST INFRASTRUCTURE ONLY.  This is the correctness judge for the HIP
- * product library; it is never linked into, loaded by or called from the
- * product path.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * TEST INFRASTRUCTURE ONLY.  The tests compare the HIP product library's
+ * tables with its results; it is never linked into, loaded by or called from
+ * the product path.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
  * baseline leg use it.
  *
  * It is a double-precision restatement of the reference GLSL passes in

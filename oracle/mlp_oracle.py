@@ -1,7 +1,8 @@
 """NumPy restatement of the surrogate MLP (include/atmo_surrogate.h):
 forward, mean-square loss and backward.
 
-TEST INFRASTRUCTURE ONLY: the judge of the GPU forward/backward kernels.
+TEST INFRASTRUCTURE ONLY: the reference the GPU forward/backward kernels are
+tested against.
 With emulate_bf16=True the activations, the hidden-layer weights and the
 hidden-layer gradients are rounded to bfloat16 at the points where the GPU
 kernels store them, so the two agree up to accumulation order.
