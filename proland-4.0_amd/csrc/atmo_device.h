@@ -34,6 +34,7 @@ struct SetDesc {
     double Rg, Rt, RL, HR, HM, mieG, alpha;
     double H;            // sqrt(Rt^2 - Rg^2)
     double Rg2, Rt2, RL2;
+    double tMu;          // tan(1.5) / 1.15  (getTransmittanceUV, FP64 lookups)
     float bR[3], bMS[3], bME[3];
     // FP32 constants
     float f_Rg, f_dRt, f_inv_dRt;   // Rg, Rt - Rg, 1/(Rt - Rg)
@@ -157,6 +158,18 @@ __device__ __forceinline__ float T_umu_t(const SetDesc &s, float mu) {
 }
 __device__ __forceinline__ float T_umu(const SetDesc &s, float mu) {
     return s.linT ? T_umu_t<true>(s, mu) : T_umu_t<false>(s, mu);
+}
+
+// The same T coordinates in FP64 (the sun's lookups of the single-scattering
+// pass: next to the horizon T is steep, and an FP32 coordinate's ~3e-8 moves a
+// grazing sun's T by ~1e-4).
+__device__ __forceinline__ double T_ur_d(const SetDesc &s, double h) {
+    const double hc = fmax(h, 0.0), dRt = s.Rt - s.Rg;
+    return s.linT ? hc / dRt : sqrt(hc / dRt);
+}
+template <bool LIN>
+__device__ __forceinline__ double T_umu_d(const SetDesc &s, double mu) {
+    return LIN ? (mu + 0.15) / 1.15 : atan((mu + 0.15) * s.tMu) / 1.5;
 }
 
 // common.glsl:223-226: T(r, mu), bilinear.

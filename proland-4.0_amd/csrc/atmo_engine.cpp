@@ -165,7 +165,8 @@ void fill_desc(SetDesc &s, const atmo_params &p) {
     s.f_inv_dRt = (float)(1.0 / (s.Rt - s.Rg));
     s.f_H = (float)s.H;
     s.f_invH = (float)(1.0 / s.H);
-    s.f_tMu = (float)(std::tan(1.5) / 1.15);
+    s.tMu = std::tan(1.5) / 1.15;
+    s.f_tMu = (float)s.tMu;
     s.f_tMuS = (float)std::tan(1.26 * 1.1);
     s.kR = (float)(3.0 / (16.0 * kPI));
     double g = s.mieG;

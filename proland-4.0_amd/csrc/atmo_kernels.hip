@@ -118,15 +118,15 @@ __device__ __forceinline__ int sn_rows_per_wg(int W4, int rpw_cap) {
 struct S1Entry {
     double ti;       // sample distance
     double q;        // r_i^2 - Rg^2
+    double iri;      // 1 / r_i
     float T[3];      // T(r, mu, t_i), trapezoid weight folded in
-    float iri;       // 1 / r_i (FP32)
     float eR, eM;    // exp(-h_i/HR), exp(-h_i/HM)
     float b;         // T-table row weight of r_i
     int j0, j1;      // T-table row offsets (texels) of r_i
-    int pad;
 };
 
 constexpr int kS1Chunk = 64;
+constexpr int kXkMax = 2048;   // T-table columns whose centres are staged for the sun lookups
 
 template <bool LINT>   // linear transmittance parameterisation (the per-sample T lookup)
 __global__ __launch_bounds__(256) void k_inscatter1(const SetDesc *__restrict__ sets,
@@ -134,8 +134,22 @@ __global__ __launch_bounds__(256) void k_inscatter1(const SetDesc *__restrict__ 
                                                     int nsamp) {
     __shared__ S1Entry ent[kNRows][kS1Chunk];
     __shared__ float rowdx[kNRows];
+    __shared__ double xk[LINT ? 1 : kXkMax];
     const uint2 tl = tiles[blockIdx.x];
     const SetDesc &s = sets[tl.x];
+    // The sun's T lookups at mu_s_i next to the horizon, where T is steep:
+    // an FP32 texel coordinate (atanf, then * TW) is good to ~3e-5 of a texel
+    // at TW = 512, which moves a grazing sun's T by ~1e-4.  So the filter
+    // fraction is taken from the centre of the column the FP32 coordinate
+    // falls in, u - u_k = (atan x - atan x_k) / 1.5 with
+    // atan x - atan x_k = atan((x - x_k) / (1 + x x_k)) (less than 1.5 / TW,
+    // FP32 is exact enough there), x = (mu_s_i + 0.15) tan(1.5) / 1.15 in FP64
+    // and x_k = tan(1.5 (k + 0.5) / TW) staged once per workgroup (tables of at
+    // most kXkMax columns; wider ones keep the FP32 coordinate).
+    const bool tab = !LINT && s.TW <= kXkMax;
+    const float sc = (float)s.TW / 1.5f;
+    if (tab)
+        for (int k = threadIdx.x; k < s.TW; k += blockDim.x) xk[k] = tan(1.5 * (k + 0.5) / s.TW);
     const int RPW = sn_rows_per_wg(s.W4, kNRows);
     const int ngy = (s.RMU + RPW - 1) / RPW;
     const int l = (int)tl.y / ngy;
@@ -192,16 +206,18 @@ __global__ __launch_bounds__(256) void k_inscatter1(const SetDesc *__restrict__ 
                                         : T_ratio_lin(s, h1, -mu1, (float)(r - s.Rg), (float)-mu);
             const float w = (i == 0 || i == nsamp) ? 0.5f : 1.0f;
             en.T[0] = Tseg.x * w; en.T[1] = Tseg.y * w; en.T[2] = Tseg.z * w;
-            en.iri = 1.0f / ri;
+            const double rid = sqrt(en.q + s.Rg2);
+            en.iri = 1.0 / rid;
             const float hc = fmaxf(h1, 0.0f);                          // ri = max(Rg, ri)
             en.eR = __expf(-hc * iHR);
             en.eM = __expf(-hc * iHM);
-            // r axis of T(max(Rg, ri), muSi): common.glsl:114-124
+            // r axis of T(max(Rg, ri), muSi): common.glsl:114-124 (FP64 coordinate)
             int j0, j1;
-            gl_axis(T_ur(s, hc) * s.TH - 0.5f, s.TH, j0, j1, en.b);
+            double bd;
+            gl_axis_dd(T_ur_d(s, en.q / (rid + s.Rg)) * s.TH - 0.5, s.TH, j0, j1, bd);
+            en.b = (float)bd;
             en.j0 = j0 * s.TW;
             en.j1 = j1 * s.TW;
-            en.pad = 0;
             ent[row][js] = en;
         }
         __syncthreads();
@@ -218,10 +234,23 @@ __global__ __launch_bounds__(256) void k_inscatter1(const SetDesc *__restrict__ 
                 const double num = fma(nud[j], en.ti, muSr[j]);        // ri * muSi
                 // muSi >= -sqrt(1 - Rg^2 / max(Rg, ri)^2)  <=>  num >= 0 or num^2 <= ri^2 - Rg^2
                 if (!(num >= 0.0 || num * num <= en.q)) continue;
-                const float muSi = (float)num * en.iri;
+                const double mus = num * en.iri;   // mu_s_i
                 int i0, i1;
                 float a;
-                gl_axis(T_umu_t<LINT>(s, muSi) * s.TW - 0.5f, s.TW, i0, i1, a);
+                if (tab) {
+                    gl_axis(T_umu_t<false>(s, (float)mus) * s.TW - 0.5f, s.TW, i0, i1, a);
+                    if (i1 == i0 + 1) {   // between two columns (not clamped to an edge)
+                        const double x = (mus + 0.15) * s.tMu, x0 = xk[i0];
+                        const float d = (float)(x - x0) * __builtin_amdgcn_rcpf(fmaf((float)x, (float)x0, 1.0f));
+                        a = atanf(d) * sc;
+                    }
+                } else if (LINT) {   // (mu_s_i + 0.15) / 1.15: FP64 is cheap here
+                    double ad;
+                    gl_axis_dd(T_umu_d<true>(s, mus) * s.TW - 0.5, s.TW, i0, i1, ad);
+                    a = (float)ad;
+                } else {   // more than kXkMax columns: the FP32 coordinate
+                    gl_axis(T_umu_t<false>(s, (float)mus) * s.TW - 0.5f, s.TW, i0, i1, a);
+                }
                 const float3 t00 = ld3(T4 + en.j0 + i0), t01 = ld3(T4 + en.j0 + i1);
                 const float3 t10 = ld3(T4 + en.j1 + i0), t11 = ld3(T4 + en.j1 + i1);
                 const float3 Ts = lerp3(lerp3(t00, t01, a), lerp3(t10, t11, a), en.b);
