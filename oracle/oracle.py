@@ -3,6 +3,7 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 CPU-baseline leg of bench.py, never by the product library.
 """
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -26,61 +27,80 @@ def build():
 _lib = None
 
 
+def load(path):
+    """Loads an oracle library built from oracle/atmo_oracle.cpp (the default
+    one, or a mutated copy a test builds) with every signature declared."""
+    L = ctypes.CDLL(path)
+    sig = {
+        "oracle_layer": (None, [_pp, ctypes.c_int, _dp, _dp]),
+        "oracle_limit": (ctypes.c_double, [_pp, ctypes.c_double, ctypes.c_double]),
+        "oracle_phaseR": (ctypes.c_double, [ctypes.c_double]),
+        "oracle_phaseM": (ctypes.c_double, [_pp, ctypes.c_double]),
+        "oracle_transmittance_uv": (None, [_pp, _po, ctypes.c_double, ctypes.c_double, _dp]),
+        "oracle_transmittance_rmu": (None, [_pp, _po, ctypes.c_int, ctypes.c_int, _dp]),
+        "oracle_irradiance_rmus": (None, [_pp, ctypes.c_int, ctypes.c_int, _dp]),
+        "oracle_mumusnu": (None, [_pp, _po, ctypes.c_double, _dp, ctypes.c_int, ctypes.c_int, _dp]),
+        "oracle_texture4D_coords": (None, [_pp, _po] + [ctypes.c_double] * 4 + [_dp]),
+        "oracle_texture4D_coords_literal": (None, [_pp, _po] + [ctypes.c_double] * 4 + [_dp]),
+        "oracle_sample2D": (None, [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_double, _dp]),
+        "oracle_sample3D": (None, [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]),
+        "oracle_texture4D": (None, [_pp, _po, _dp, ctypes.c_int] + [ctypes.c_double] * 4 + [_dp]),
+        "oracle_transmittance_lookup": (None, [_pp, _dp, ctypes.c_double, ctypes.c_double, _dp]),
+        "oracle_transmittance_segment": (None, [_pp, _dp, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_double, _dp]),
+        "oracle_transmittance_segment_tau": (None, [_pp, _po, _dp, ctypes.c_double,
+                                                    ctypes.c_double, ctypes.c_double, _dp]),
+        "oracle_optical_depth_table": (None, [_pp, _po, _dp]),
+        "oracle_optical_depth": (ctypes.c_double, [_pp, _po, ctypes.c_double, ctypes.c_double,
+                                                   ctypes.c_double]),
+        "oracle_pass_transmittance": (None, [_pp, _po, _dp]),
+        "oracle_pass_irradiance1": (None, [_pp, _po, _dp, _dp]),
+        "oracle_pass_inscatter1": (None, [_pp, _po, _dp, _dp, _dp]),
+        "oracle_pass_copy_inscatter1": (None, [_pp, _dp, _dp, _dp]),
+        "oracle_pass_inscatterS": (None, [_pp, _po, _dp, _dp, _dp, _dp, ctypes.c_int, _dp]),
+        "oracle_pass_irradianceN": (None, [_pp, _po, _dp, _dp, ctypes.c_int, _dp]),
+        "oracle_pass_inscatterN": (None, [_pp, _po, _dp, _dp, _dp]),
+        "oracle_pass_copy_inscatterN": (None, [_pp, _dp, _dp]),
+        "oracle_pass_copy_irradiance": (None, [_pp, ctypes.c_double, _dp, _dp]),
+        "oracle_precompute": (None, [_pp, _po, _dp, _dp, _dp]),
+        "oracle_q16": (ctypes.c_double, [ctypes.c_double]),
+        "oracle_sky_radiance": (None, [_pp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp]),
+        "oracle_in_scattering": (None, [_pp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double,
+                                        ctypes.c_double, _dp, _dp]),
+        "oracle_sun_sky_irradiance": (None, [_pp, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                             _dp, _dp]),
+        "oracle_set_threads": (None, [ctypes.c_int]),
+        "oracle_get_threads": (ctypes.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = ctypes.CDLL(LIB)
-        sig = {
-            "oracle_layer": (None, [_pp, ctypes.c_int, _dp, _dp]),
-            "oracle_limit": (ctypes.c_double, [_pp, ctypes.c_double, ctypes.c_double]),
-            "oracle_phaseR": (ctypes.c_double, [ctypes.c_double]),
-            "oracle_phaseM": (ctypes.c_double, [_pp, ctypes.c_double]),
-            "oracle_transmittance_uv": (None, [_pp, _po, ctypes.c_double, ctypes.c_double, _dp]),
-            "oracle_transmittance_rmu": (None, [_pp, _po, ctypes.c_int, ctypes.c_int, _dp]),
-            "oracle_irradiance_rmus": (None, [_pp, ctypes.c_int, ctypes.c_int, _dp]),
-            "oracle_mumusnu": (None, [_pp, _po, ctypes.c_double, _dp, ctypes.c_int, ctypes.c_int, _dp]),
-            "oracle_texture4D_coords": (None, [_pp, _po] + [ctypes.c_double] * 4 + [_dp]),
-            "oracle_texture4D_coords_literal": (None, [_pp, _po] + [ctypes.c_double] * 4 + [_dp]),
-            "oracle_sample2D": (None, [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                       ctypes.c_double, ctypes.c_double, _dp]),
-            "oracle_sample3D": (None, [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                       ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]),
-            "oracle_texture4D": (None, [_pp, _po, _dp, ctypes.c_int] + [ctypes.c_double] * 4 + [_dp]),
-            "oracle_transmittance_lookup": (None, [_pp, _dp, ctypes.c_double, ctypes.c_double, _dp]),
-            "oracle_transmittance_segment": (None, [_pp, _dp, ctypes.c_double, ctypes.c_double,
-                                                    ctypes.c_double, _dp]),
-            "oracle_transmittance_segment_tau": (None, [_pp, _po, _dp, ctypes.c_double,
-                                                        ctypes.c_double, ctypes.c_double, _dp]),
-            "oracle_optical_depth_table": (None, [_pp, _po, _dp]),
-            "oracle_optical_depth": (ctypes.c_double, [_pp, _po, ctypes.c_double, ctypes.c_double,
-                                                       ctypes.c_double]),
-            "oracle_pass_transmittance": (None, [_pp, _po, _dp]),
-            "oracle_pass_irradiance1": (None, [_pp, _po, _dp, _dp]),
-            "oracle_pass_inscatter1": (None, [_pp, _po, _dp, _dp, _dp]),
-            "oracle_pass_copy_inscatter1": (None, [_pp, _dp, _dp, _dp]),
-            "oracle_pass_inscatterS": (None, [_pp, _po, _dp, _dp, _dp, _dp, ctypes.c_int, _dp]),
-            "oracle_pass_irradianceN": (None, [_pp, _po, _dp, _dp, ctypes.c_int, _dp]),
-            "oracle_pass_inscatterN": (None, [_pp, _po, _dp, _dp, _dp]),
-            "oracle_pass_copy_inscatterN": (None, [_pp, _dp, _dp]),
-            "oracle_pass_copy_irradiance": (None, [_pp, ctypes.c_double, _dp, _dp]),
-            "oracle_precompute": (None, [_pp, _po, _dp, _dp, _dp]),
-            "oracle_q16": (ctypes.c_double, [ctypes.c_double]),
-            "oracle_sky_radiance": (None, [_pp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp]),
-            "oracle_in_scattering": (None, [_pp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double,
-                                            ctypes.c_double, _dp, _dp]),
-            "oracle_sun_sky_irradiance": (None, [_pp, _dp, _dp, ctypes.c_double, ctypes.c_double,
-                                                 _dp, _dp]),
-            "oracle_set_threads": (None, [ctypes.c_int]),
-            "oracle_get_threads": (ctypes.c_int, []),
-        }
-        for name, (res, args) in sig.items():
-            f = getattr(L, name)
-            f.restype = res
-            f.argtypes = args
-        _lib = L
+        _lib = load(LIB)
     return _lib
+
+
+@contextlib.contextmanager
+def using(path):
+    """Routes every function of this module through the library at path
+    (a mutated oracle built by a test) for the duration of the block."""
+    global _lib
+    saved = lib()
+    _lib = load(path)
+    try:
+        yield
+    finally:
+        _lib = saved
 
 
 def _d(a):

@@ -7,6 +7,8 @@ inverse-function round trips and brute force, chosen so that a dropped term,
 a wrong sign/index or a swapped operand fails at least one of them.
 """
 import math
+import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -545,3 +547,143 @@ def test_segment_ratio_from_optical_depths_beyond_fp64_range(P):
                 beyond += 1
                 assert 0.0 < got[c] < 1.0
     assert beyond >= 4
+
+
+# ---------------------------------------------------------------- segment transmittance
+def _segment_tables(P):
+    """A fine T table (1024x256, 2000 samples per optical depth) and its depth
+    table, so that the lookups' own error (bilinear between texels, trapezoid)
+    stays below 2.2e-3 of the optical depth on every segment below."""
+    q = P.copy()
+    q.TRANSMITTANCE_W, q.TRANSMITTANCE_H = 1024, 256
+    o = options(transmittance_samples=2000)
+    return q, o, O.transmittance(q, o), O.optical_depth_table(q, o)
+
+
+def _segment_exact_depth(q, r, mu, d):
+    """Optical depth of the segment from radius r along mu over length d:
+    sum_i beta_i * int_0^d exp(-(|x + t v| - Rg) / H_i) dt by adaptive quadrature."""
+    bR = np.array([q.betaR.x, q.betaR.y, q.betaR.z])
+    bM = np.array([q.betaMEx.x, q.betaMEx.y, q.betaMEx.z])
+
+    def rho(t, H):
+        return math.exp(-(math.sqrt(r * r + t * t + 2.0 * r * mu * t) - q.Rg) / H)
+
+    iR = integrate.quad(lambda t: rho(t, q.HR), 0, d, epsabs=0, epsrel=1e-12, limit=200)[0]
+    iM = integrate.quad(lambda t: rho(t, q.HM), 0, d, epsabs=0, epsrel=1e-12, limit=200)[0]
+    return bR * iR + bM * iM
+
+
+def _segment_cases(q):
+    """Upward and downward segments at three altitudes, none reaching the
+    ground: 10, 40 and 90 % of the ray length limit(r, mu), including grazing
+    rays just above the horizon that pass their lowest point and rise again."""
+    out = []
+    for alt in (0.5, 5.0, 30.0):
+        r = q.Rg + alt
+        muh = -math.sqrt(1.0 - (q.Rg / r) ** 2)
+        for mu in (0.9, 0.3, 0.05, -0.05, muh + 0.02, -0.5):
+            L = O.limit(q, r, mu)
+            for f in (0.1, 0.4, 0.9):
+                out.append((r, mu, f * L))
+    return out
+
+
+_SEGMENT_CASES = None
+
+
+def _segment_worst(P):
+    """Worst |ln T_segment + tau| / (3e-3 tau + 2e-5) over the cases, for both
+    the plain quotient of T lookups and the chain's form from optical depths
+    (D6).  <= 1 passes; the lookups' own error reaches 2.2e-3 tau."""
+    global _SEGMENT_CASES
+    if _SEGMENT_CASES is None:
+        q, o, T, tau = _segment_tables(P)
+        _SEGMENT_CASES = (q, o, T, tau, [(r, mu, d, _segment_exact_depth(q, r, mu, d))
+                                         for (r, mu, d) in _segment_cases(q)])
+    q, o, T, tau, cases = _SEGMENT_CASES
+    worst = 0.0
+    for (r, mu, d, ex) in cases:
+        for got in (O.transmittance_segment(q, T, r, mu, d),
+                    O.transmittance_segment_tau(q, tau, r, mu, d, o)):
+            with np.errstate(divide="ignore"):
+                depth = -np.log(got)
+            err = np.max(np.abs(depth - ex) / (3e-3 * ex + 2e-5))
+            worst = max(worst, float(err) if np.isfinite(err) else math.inf)
+    return worst
+
+
+def test_segment_transmittance_matches_quadrature(P):
+    """common.glsl:272-282: T(x, x + d v) = min(T(r, mu) / T(r1, mu1), 1) for
+    mu > 0 and T(r1, -mu1) / T(r, -mu) otherwise, against exp(-optical depth)
+    of the segment by quadrature, with T tables that are not identically 1."""
+    assert _segment_worst(P) <= 1.0
+
+
+# ---------------------------------------------------------------- copyInscatterN
+def _copy_inscatterN_worst(P):
+    """copyInscatterN.glsl:105-110 with the ADD blend (PreprocessAtmo.cpp:336,350):
+    with dS = c * phaseR(nu_k), nu_k = -1 + 2k / (RES_NU - 1) the nu of column
+    x = k RES_MU_S + j (common.glsl:199), S.rgb gains exactly c and S.a gains 0.
+    Returns the worst relative deviation."""
+    p = tiny(P, R=4, MU=16, MUS=8, NU=5)
+    rng = np.random.default_rng(11)
+    W = p.RES_NU * p.RES_MU_S
+    c = rng.uniform(0.1, 2.0, (p.RES_R, p.RES_MU, W, 3))
+    k = np.arange(W) // p.RES_MU_S
+    nu = -1.0 + 2.0 * k / (p.RES_NU - 1.0)
+    pR = 3.0 / (16.0 * PI_REF) * (1.0 + nu * nu)
+    dS = np.ascontiguousarray(c * pR[None, None, :, None])
+    S0 = rng.uniform(0.0, 1.0, (p.RES_R, p.RES_MU, W, 4))
+    S1 = O.copy_inscatterN(p, dS, S0)
+    rgb = np.max(np.abs((S1[..., :3] - S0[..., :3]) - c) / c)
+    alpha = np.max(np.abs(S1[..., 3] - S0[..., 3]))
+    return max(float(rgb), float(alpha))
+
+
+def test_copy_inscatterN_divides_by_rayleigh_phase_of_the_texel_nu(P):
+    assert _copy_inscatterN_worst(P) <= 1e-13
+
+
+# ---------------------------------------------------------------- planted mistakes
+_ORACLE_SRC = os.path.join(os.path.dirname(O.__file__), "atmo_oracle.cpp")
+
+# (pin, function whose body is mutated, text replaced, replacement)
+_MUTANTS = {
+    "segment branches swapped": (_segment_worst, "void T_segment(", "if (mu > 0.0)", "if (!(mu > 0.0))"),
+    "segment end radius sign": (_segment_worst, "void T_segment(", "+ 2.0 * r * mu * d", "- 2.0 * r * mu * d"),
+    "segment end cosine sign": (_segment_worst, "void T_segment(", "(r * mu + d) / r1", "(r * mu - d) / r1"),
+    "copyInscatterN times phase": (_copy_inscatterN_worst, "void oracle_pass_copy_inscatterN(",
+                                   "dS[k * 3 + c] / pr", "dS[k * 3 + c] * pr"),
+    "copyInscatterN into alpha": (_copy_inscatterN_worst, "void oracle_pass_copy_inscatterN(",
+                                  "S[k * 4 + 3] += 0.0", "S[k * 4 + 3] += dS[k * 3] / pr"),
+}
+_PIN_LIMIT = {_segment_worst: 1.0, _copy_inscatterN_worst: 1e-13}
+
+
+def _build_mutant(tmp_path, func, old, new):
+    src = open(_ORACLE_SRC).read()
+    a = src.index(func)
+    b = src.index("\n}\n", a)
+    body = src[a:b]
+    assert old in body, (func, old)
+    mutated = src[:a] + body.replace(old, new) + src[b:]
+    cpp = os.path.join(str(tmp_path), "atmo_oracle.cpp")
+    so = os.path.join(str(tmp_path), "liboracle_mutant.so")
+    with open(cpp, "w") as f:
+        f.write(mutated)
+    subprocess.check_call(["g++", "-O1", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off",
+                           "-shared", "-I", os.path.dirname(_ORACLE_SRC), "-o", so, cpp])
+    return so
+
+
+@pytest.mark.parametrize("name", sorted(_MUTANTS))
+def test_pins_fail_on_planted_mistakes(P, tmp_path, name):
+    """Each pin above rejects an oracle with the mistake planted (a mutated copy
+    of oracle/atmo_oracle.cpp built in a temporary directory), while the
+    oracle itself passes it."""
+    pin, func, old, new = _MUTANTS[name]
+    assert pin(P) <= _PIN_LIMIT[pin]
+    so = _build_mutant(tmp_path, func, old, new)
+    with O.using(so):
+        assert pin(P) > _PIN_LIMIT[pin], name
