@@ -145,31 +145,33 @@ def main():
     # lookups; the third argument is the staging-in-rounds variant)
     later = [ms for name, order, ms in times if name == "inscatterS" and order >= 3]
     avg_ms = sum(later) / len(later)
-    flop_per_launch = p.n_texels4d() * F.inscatterS_texel(o.spherical_samples, False)
-    achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12
     kname = "k_inscatterS<false,%d,false>" % eng.tuning().get("inscatterS_texels_per_thread", 4)
     traffic = profiled(TRAFFIC_PROFILE, "traffic", kname)
+    # FP32 FLOPs one launch executes (rocprofv3 SQ_INSTS_VALU_FLOPS_FP32 x 64 lanes,
+    # FMA = 2) over its live launch time: the roofline figure
     executed = profiled(COUNTER_PROFILE, "executed_fp32_flop", kname)
-    roof = {"bound": "valu", "achieved": round(achieved, 2), "peak": FP32_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+    # the same launch counted in the reference's formulation (flops.py: its GLSL,
+    # each add/mul/transcendental 1, a GL trilinear fetch 75); the kernel stages the
+    # separable part of each lookup once per workgroup, so it executes fewer
+    formulation = p.n_texels4d() * F.inscatterS_texel(o.spherical_samples, False)
+    achieved = executed / (avg_ms * 1e-3) / 1e12 if executed else None
+    roof = {"bound": "valu",
+            "achieved": round(achieved, 2) if achieved else None, "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4) if achieved else None,
             "traffic": traffic,
             "kernel": "%s (dJ pass, orders 3-%d)" % (kname, o.orders),
             "avg_launch_ms": round(avg_ms, 4), "launches": len(later),
-            "flop_per_launch": flop_per_launch,
-            "flop_convention": "reference GLSL formulation (prolandatmo/flops.py); the kernel "
-                               "executes fewer FP32 operations than this (separable filtering "
-                               "staged once per workgroup)"}
+            "flop_per_launch": executed,
+            "flop_source": "executed FP32 FLOPs per launch: rocprofv3 SQ_INSTS_VALU_FLOPS_FP32 x 64 "
+                           "(%s), over the launch time measured here with hipEvents" % COUNTER_PROFILE,
+            # packed FP32 (v_pk_fma_f32) is disabled in this build; one FMA per lane per clock
+            "frac_of_unpacked_fp32_peak": (round(achieved / FP32_UNPACKED_PEAK_TFLOPS, 4)
+                                           if achieved else None),
+            "formulation_flop_per_launch": formulation,
+            "formulation_tflops": round(formulation / (avg_ms * 1e-3) / 1e12, 2),
+            "formulation_speedup": round(formulation / executed, 2) if executed else None}
     if traffic:
         roof["traffic_source"] = TRAFFIC_PROFILE
-    if executed:
-        # executed FP32 FLOPs per launch (SQ_INSTS_VALU_FLOPS_FP32 x 64) over the live time
-        roof["executed_fp32_tflops"] = round(executed / (avg_ms * 1e-3) / 1e12, 2)
-        roof["executed_frac"] = round(executed / (avg_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)
-        roof["executed_source"] = COUNTER_PROFILE
-        # the 157.3 peak counts packed FP32, which this build does not emit
-        roof["executed_frac_of_unpacked_fp32_peak"] = round(
-            executed / (avg_ms * 1e-3) / 1e12 / FP32_UNPACKED_PEAK_TFLOPS, 4)
-    if traffic:
         roof["hbm_gbs"] = round(traffic / (avg_ms * 1e-3) / 1e9, 1)
         roof["hbm_frac"] = round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     whole_tflops = F.total(p, o) / sec / 1e12

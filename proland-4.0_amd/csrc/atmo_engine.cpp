@@ -585,12 +585,15 @@ atmo_status launch_all(Plan &pl, hipStream_t st, PassTimer *tm = nullptr) {
 // ---------------------------------------------------------------------------
 // Autotuner.  After the first run of a plan whose shape class has not been
 // tuned in this process, every variant of the two dominant passes is timed on
-// the plan's own (by then scratch) work tables: k_inscatterS with 2 or 4
-// texels per thread (orders 2 and >= 3), k_inscatterN with up to 1, 2 or 4 mu
-// rows per workgroup, with and without the XCD-banded tile order (its
+// the plan's own (by then scratch) work tables: k_inscatterS with 2, 3, 4 or 8
+// texels per thread (orders 2 and >= 3), k_inscatterN in its three forms with
+// up to 1, 2 or 4 mu rows per workgroup and three tile orders (its
 // descriptors carry no S pointer, so the finished tables are not touched).
-// All variants compute bit-identical results; the fastest is kept per shape
-// class and the plan's graph is re-captured.
+// The k_inscatterS variants compute bit-identical tables; the k_inscatterN
+// forms (direct gathers, LDS-staged rows) evaluate the same filter in a
+// different FP32 order.  The fastest is kept per shape class, and the first
+// call then runs the passes again with it, so that every call on a shape class
+// returns the tables of the same variants.
 // ---------------------------------------------------------------------------
 std::string shape_key(const std::vector<atmo_params> &ps, const atmo_options &o) {
     std::vector<std::string> parts;
@@ -810,6 +813,57 @@ atmo_status build_plan(atmo_engine *e, const atmo_params *sets, int nsets, const
     return ATMO_OK;
 }
 
+// Runs every pass of a plan on the caller's stream: with pass timing (eager
+// launches between hipEvent pairs), eagerly, or by replaying the plan's graph
+// (captured on first use).
+atmo_status run_plan(atmo_engine *e, Plan &pl, const atmo_options &o, hipStream_t hs) {
+    atmo_status st;
+    if (o.flags & ATMO_FLAG_TIME_PASSES) {
+        PassTimer tm;
+        tm.on = true;
+        if ((st = launch_all(pl, hs, &tm)) != ATMO_OK) return st;
+        HIPCHK(hipStreamSynchronize(hs));
+        e->times.clear();
+        for (auto &r : tm.recs) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+            e->times.push_back({r.pass, r.order, ms});
+        }
+    } else if (o.flags & ATMO_FLAG_NO_GRAPH) {
+        if ((st = launch_all(pl, hs)) != ATMO_OK) return st;
+    } else {
+        bool eager = false;
+        if (!pl.gexec) {
+            HIPCHK(hipStreamBeginCapture(e->cap, hipStreamCaptureModeThreadLocal));
+            atmo_status cs = launch_all(pl, e->cap);
+            hipGraph_t g = nullptr;
+            hipError_t ce = hipStreamEndCapture(e->cap, &g);
+            if (ce == hipErrorStreamCaptureInvalidated) {
+                // another thread's work on the legacy stream can invalidate a
+                // capture in progress: run this call's launches directly and
+                // capture again on the next call
+                if (g) (void)hipGraphDestroy(g);
+                (void)hipGetLastError();
+                eager = true;
+            } else {
+                if (cs != ATMO_OK) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return cs;
+                }
+                HIPCHK(ce);
+                pl.graph = g;
+                HIPCHK(hipGraphInstantiate(&pl.gexec, g, nullptr, nullptr, 0));
+            }
+        }
+        if (eager) {
+            if ((st = launch_all(pl, hs)) != ATMO_OK) return st;
+        } else {
+            HIPCHK(hipGraphLaunch(pl.gexec, hs));
+        }
+    }
+    return ATMO_OK;
+}
+
 atmo_status check_out(const atmo_tables &t, int i) {
     if (!t.transmittance || !t.irradiance || !t.inscatter)
         return fail(ATMO_ERR_INVALID_ARGUMENT, "output tables of set %d: NULL pointer", i);
@@ -952,51 +1006,12 @@ atmo_status atmo_engine_precompute(atmo_engine *e, const atmo_params *sets, int 
         HIPCHK(hipStreamSynchronize(hs));
         HIPCHK(h2d(pl.dsets.p, pl.hsets.data(), nsets * sizeof(SetDesc), e->cap));
     }
-    if (o.flags & ATMO_FLAG_TIME_PASSES) {
-        PassTimer tm;
-        tm.on = true;
-        if ((st = launch_all(pl, hs, &tm)) != ATMO_OK) return st;
-        HIPCHK(hipStreamSynchronize(hs));
-        e->times.clear();
-        for (auto &r : tm.recs) {
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
-            e->times.push_back({r.pass, r.order, ms});
-        }
-    } else if (o.flags & ATMO_FLAG_NO_GRAPH) {
-        if ((st = launch_all(pl, hs)) != ATMO_OK) return st;
-    } else {
-        bool eager = false;
-        if (!pl.gexec) {
-            HIPCHK(hipStreamBeginCapture(e->cap, hipStreamCaptureModeThreadLocal));
-            atmo_status cs = launch_all(pl, e->cap);
-            hipGraph_t g = nullptr;
-            hipError_t ce = hipStreamEndCapture(e->cap, &g);
-            if (ce == hipErrorStreamCaptureInvalidated) {
-                // another thread's work on the legacy stream can invalidate a
-                // capture in progress: run this call's launches directly and
-                // capture again on the next call
-                if (g) (void)hipGraphDestroy(g);
-                (void)hipGetLastError();
-                eager = true;
-            } else {
-                if (cs != ATMO_OK) {
-                    if (g) (void)hipGraphDestroy(g);
-                    return cs;
-                }
-                HIPCHK(ce);
-                pl.graph = g;
-                HIPCHK(hipGraphInstantiate(&pl.gexec, g, nullptr, nullptr, 0));
-            }
-        }
-        if (eager) {
-            if ((st = launch_all(pl, hs)) != ATMO_OK) return st;
-        } else {
-            HIPCHK(hipGraphLaunch(pl.gexec, hs));
-        }
-    }
+    // A plan that has not been tuned takes the cached tuning of its shape class
+    // before its first launch; with none cached, it is tuned after this run.
+    bool tune_after = false;
+    std::string key;
     if (!pl.tuned && (o.orders >= 2) && !(o.flags & ATMO_FLAG_NO_AUTOTUNE)) {
-        const std::string key = shape_key(pl.params, o);
+        key = shape_key(pl.params, o);
         Tuning tu;
         bool have = false;
         {
@@ -1007,14 +1022,27 @@ atmo_status atmo_engine_precompute(atmo_engine *e, const atmo_params *sets, int 
                 have = true;
             }
         }
-        if (!have) {
-            if ((st = autotune(pl, hs, tu)) != ATMO_OK) return st;
+        if (have) {
+            if ((st = apply_tuning(pl, tu)) != ATMO_OK) return st;
+            e->last_tuning = tu;
+            e->last_key = key;
+        } else {
+            tune_after = true;
+        }
+    }
+    if ((st = run_plan(e, pl, o, hs)) != ATMO_OK) return st;
+    if (tune_after) {
+        Tuning tu;
+        if ((st = autotune(pl, hs, tu)) != ATMO_OK) return st;
+        {
             std::lock_guard<std::mutex> lk(tuning_mutex());
             tuning_cache()[key] = tu;
         }
         if ((st = apply_tuning(pl, tu)) != ATMO_OK) return st;
         e->last_tuning = tu;
         e->last_key = key;
+        // the tables of this call from the variants every later call runs
+        if ((st = run_plan(e, pl, o, hs)) != ATMO_OK) return st;
     }
     if (o.flags & ATMO_FLAG_CHECK_NUMERICS) {
         HIPCHK(hipMemsetAsync(pl.dflags.p, 0, nsets * sizeof(unsigned), hs));
@@ -1050,7 +1078,8 @@ atmo_status atmo_precompute_host(const atmo_params *params, const atmo_options *
     t.irradiance = (float *)((char *)out.p + bT);
     t.inscatter = (float *)((char *)out.p + bT + bE);
     atmo_options o = opts ? *opts : default_options();
-    o.flags |= ATMO_FLAG_NO_GRAPH;
+    // a one-off engine: no graph to replay and no later call to tune for
+    o.flags |= ATMO_FLAG_NO_GRAPH | ATMO_FLAG_NO_AUTOTUNE;
     if ((st = atmo_engine_precompute(e, params, 1, &o, &t, nullptr)) != ATMO_OK) return st;
     HIPCHK(hipDeviceSynchronize());
     if (T) HIPCHK(hipMemcpy(T, t.transmittance, z.nT * 12, hipMemcpyDeviceToHost));

@@ -545,6 +545,26 @@ def test_largest_tables_the_abi_accepts(eng):
     assert torch.equal(S, S0)
 
 
+def test_first_and_later_calls_return_the_same_bits():
+    """A shape class not tuned before in the process: the first call (which
+    runs, tunes, and runs again with the chosen variants), a replay of the
+    tuned graph on the same engine, and a new engine (which applies the cached
+    tuning before its first launch) return the same tables bit for bit."""
+    p = small(res=(6, 24, 6, 4), T=(32, 8), E=(8, 4))
+    a = L.Engine(0)
+    first = [t.clone() for t in a.precompute(p, options())[0]]
+    second = a.precompute(p, options())[0]
+    torch.cuda.synchronize()
+    assert a.tuning()["tuned"] == 1
+    a.close()
+    b = L.Engine(0)
+    third = b.precompute(p, options())[0]
+    torch.cuda.synchronize()
+    b.close()
+    for x, y, z in zip(first, second, third):
+        assert torch.equal(x, y) and torch.equal(x, z)
+
+
 def test_golden_tiny_fixtures(eng):
     g = np.load(os.path.join(GOLDEN, "tiny_earth.npz"))
     p = _params(g["params"])
