@@ -178,9 +178,9 @@ ATMO_API atmo_status atmo_engine_pass_times(atmo_engine *e, atmo_pass_time *out,
  * times every variant of the two dominant passes and keeps the fastest.
  * k_inscatterS variants (2 / 3 / 4 / 8 texels per thread, or each set with
  * the G whose workgroup-occupancy cost ceil(RES_MU * ceil(RES_NU / G) / 256) * G
- * is lowest, reported as 0) give bit-identical tables; the two k_inscatterN
- * forms (direct gathers / LDS-staged blended rows) compute the same trilinear
- * filter in a different FP32 order.  Times in ms: inscatterS_ms[i] for
+ * is lowest, reported as 0) and the k_inscatterN forms (direct gathers /
+ * LDS-staged blended rows) give bit-identical tables: tuning changes the time
+ * only.  Times in ms: inscatterS_ms[i] for
  * 2 / 3 / 4 / 8 / per-set texels per thread (one order-2 + (orders-2) later
  * launches), inscatterN_ms[9f + 3c + x] for form f (direct, staged with 1
  * sample per LDS round, staged with 2), at most 1 / 2 / 4 mu rows per

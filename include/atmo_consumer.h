@@ -57,6 +57,42 @@ ATMO_API atmo_status atmo_sun_sky_irradiance(const atmo_params *p, const float *
                                              const float *rmus, int n, float *sun, float *sky,
                                              void *stream);
 
+/* ---------------------------------------------------------------------- */
+/* ork texture descriptors                                                 */
+/* ---------------------------------------------------------------------- */
+
+/* One texture2D / texture3D element of an ork resource file, the way the
+ * renderers load the tables: src/demo/textures/atmo/{transmittance,
+ * irradiance,inscatter}.xml:25 and src/atmo/examples/atmo/helloworld.xml:71-75
+ * (source="<table>.raw" internalformat="RGBA16F" min="LINEAR" mag="LINEAR"
+ * wraps/wrapt/wrapr="CLAMP_TO_EDGE"), with the sizes from the .raw trailer of
+ * its source (PreprocessAtmo.cpp:355-405). */
+typedef struct atmo_texture_desc {
+    char name[64];
+    char source[512];        /* the .raw path, resolved against the descriptor's directory */
+    char format[16];         /* internalformat */
+    int kind;                /* 2: texture2D, 3: texture3D */
+    int format_channels;     /* channels of the internal format */
+    int half;                /* 1: a 16F format (values stored as IEEE binary16) */
+    int min_linear, mag_linear;
+    int clamp_to_edge[3];    /* wraps, wrapt, wrapr == CLAMP_TO_EDGE */
+    int width, height, depth, components;   /* trailer: depth 0 for a 2D table */
+} atmo_texture_desc;
+
+/* Parses the first texture2D / texture3D element of the descriptor file and
+ * validates it against its source's trailer: magic and size, texture3D <=>
+ * depth > 0, a known float internalformat with at least as many channels as
+ * the payload, an explicit wrap mode on every axis. */
+ATMO_API atmo_status atmo_texture_load_desc(const char *xml_path, atmo_texture_desc *out);
+
+/* The payload of the descriptor's source (.raw layout: components floats per
+ * texel, x fastest) as the GPU texture holds it: with a 16F internalformat
+ * every value rounded to the nearest binary16 (round to nearest even, gradual
+ * underflow, +-inf from 65520), a 32F one unchanged.  data: host buffer of
+ * capacity_floats; out (may be NULL) receives the descriptor. */
+ATMO_API atmo_status atmo_texture_load(const char *xml_path, float *data, size_t capacity_floats,
+                                       atmo_texture_desc *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -132,6 +132,27 @@ LayerGeom layer_geom(const atmo_params &p, int l) {
     return g;
 }
 
+// Extra LDS entries the merged ground rows of the orders >= 3 dJ pass need in
+// one staging round of ch theta rows: the largest sum of merged_row_extra over
+// the rows of a round, over every layer of every set.
+int merged_extra(const atmo_params *sets, int nsets, int NS, int ch) {
+    int best = 0;
+    std::vector<int> ex(NS);
+    for (int i = 0; i < nsets; ++i) {
+        const atmo_params &p = sets[i];
+        for (int l = 0; l < p.RES_R; ++l) {
+            const double r = layer_geom(p, l).r;
+            for (int it = 0; it < NS; ++it) ex[it] = merged_row_extra(r, p.Rg, p.Rt, p.SKY_W, NS, it);
+            for (int c0 = 0; c0 < NS; c0 += ch) {
+                int sum = 0;
+                for (int it = c0; it < std::min(NS, c0 + ch); ++it) sum += ex[it];
+                best = std::max(best, sum);
+            }
+        }
+    }
+    return best;
+}
+
 // mu of texel row y of a layer (getMuMuSNu, common.glsl:175-190), as the kernels compute it
 double row_mu(const atmo_params &p, const LayerGeom &g, int y) {
     const double r = g.r, Rg = p.Rg, Rt = p.Rt, half = p.RES_MU * 0.5;
@@ -290,6 +311,7 @@ struct Plan {
     size_t dirJT_off = 0, dirJW_off = 0, dirE_off = 0;
     size_t ldsJ1 = 0, ldsJ = 0, ldsE1 = 0, ldsEN = 0;
     int chJ1 = 0, chJ = 0, chE1 = 0, chEN = 0;   // theta rows per LDS staging round
+    int jx = 0;   // extra LDS entries of the merged ground rows of an orders >= 3 dJ round
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     // kernel variants (picked by the autotuner)
@@ -381,16 +403,22 @@ hipError_t h2d(void *dst, const void *src, size_t n, hipStream_t xs) {
     return e != hipSuccess ? e : hipStreamSynchronize(xs);
 }
 
+// Bytes of the direction tables (build_dirs) and the offsets of their parts.
+size_t dirs_bytes(const atmo_options &o, size_t &offJT, size_t &offJW, size_t &offE) {
+    const size_t NS = o.spherical_samples, NI = o.irradiance_samples;
+    offJT = 0;
+    offJW = align256(NS * sizeof(DirJHost));
+    offE = offJW + align256(NS * 2 * NS * 2 * sizeof(float));
+    return offE + align256(2 * NI * (NI / 2) * sizeof(DirEHost));
+}
+
 atmo_status upload_dirs(DevBuf &buf, hipStream_t xs, const atmo_options &o, size_t &offJT, size_t &offJW,
                         size_t &offE) {
     std::vector<DirJHost> jt;
     std::vector<float> jw;
     std::vector<DirEHost> de;
     build_dirs(o.spherical_samples, o.irradiance_samples, jt, jw, de);
-    offJT = 0;
-    offJW = align256(jt.size() * sizeof(DirJHost));
-    offE = offJW + align256(jw.size() * sizeof(float));
-    size_t total = offE + align256(de.size() * sizeof(DirEHost));
+    const size_t total = dirs_bytes(o, offJT, offJW, offE);
     HIPCHK(buf.ensure(total));
     std::vector<char> h(total, 0);
     memcpy(h.data() + offJT, jt.data(), jt.size() * sizeof(DirJHost));
@@ -506,10 +534,10 @@ hipError_t launch_dj(const Plan &pl, const SetDesc *S, const uint2 *T, const cha
     const size_t lds = first ? pl.ldsJ1 : pl.ldsJ;
     const int ch = first ? pl.chJ1 : pl.chJ;
     if (G != 0)
-        return launch_inscatterS(S, T + pl.off[P_J], pl.cnt[P_J], o.spherical_samples, ch,
+        return launch_inscatterS(S, T + pl.off[P_J], pl.cnt[P_J], o.spherical_samples, ch, pl.jx,
                                  dirs + pl.dirJT_off, dirs + pl.dirJW_off, first, lds, G, st);
     for (const JSeg &g : pl.jseg) {
-        hipError_t e = launch_inscatterS(S, T + pl.off[P_J] + g.off, g.cnt, o.spherical_samples, ch,
+        hipError_t e = launch_inscatterS(S, T + pl.off[P_J] + g.off, g.cnt, o.spherical_samples, ch, pl.jx,
                                          dirs + pl.dirJT_off, dirs + pl.dirJW_off, first, lds, g.G, st);
         if (e != hipSuccess) return e;
     }
@@ -589,11 +617,10 @@ atmo_status launch_all(Plan &pl, hipStream_t st, PassTimer *tm = nullptr) {
 // texels per thread (orders 2 and >= 3), k_inscatterN in its three forms with
 // up to 1, 2 or 4 mu rows per workgroup and three tile orders (its
 // descriptors carry no S pointer, so the finished tables are not touched).
-// The k_inscatterS variants compute bit-identical tables; the k_inscatterN
-// forms (direct gathers, LDS-staged rows) evaluate the same filter in a
-// different FP32 order.  The fastest is kept per shape class, and the first
-// call then runs the passes again with it, so that every call on a shape class
-// returns the tables of the same variants.
+// Every variant computes bit-identical tables (the k_inscatterN forms blend the
+// same columns in the same FP32 order), so tuning changes the time only.  The
+// fastest is kept per shape class and applied before the first launch of any
+// later plan of that class.
 // ---------------------------------------------------------------------------
 std::string shape_key(const std::vector<atmo_params> &ps, const atmo_options &o) {
     std::vector<std::string> parts;
@@ -735,10 +762,10 @@ atmo_status apply_tuning(Plan &pl, const Tuning &tu) {
     return ATMO_OK;
 }
 
-atmo_status build_plan(atmo_engine *e, const atmo_params *sets, int nsets, const atmo_options &o,
+atmo_status build_plan(hipStream_t xs, const atmo_params *sets, int nsets, const atmo_options &o,
                        std::unique_ptr<Plan> &out) {
     std::unique_ptr<Plan> pl(new Plan);
-    pl->xs = e->cap;
+    pl->xs = xs;
     pl->params.assign(sets, sets + nsets);
     pl->opts = o;
     pl->jG = o.texels_per_thread;   // 0: each set with its own G until the tuner has run
@@ -787,7 +814,7 @@ atmo_status build_plan(atmo_engine *e, const atmo_params *sets, int nsets, const
     HIPCHK(pl->dsets.ensure(nsets * sizeof(SetDesc)));
     // tiles
     std::vector<uint2> tiles;
-    build_tiles(pl->params, o, tiles, pl->off, pl->cnt, 4, 1, &pl->jseg);
+    build_tiles(pl->params, o, tiles, pl->off, pl->cnt, pl->snCap, pl->snXcd, &pl->jseg);
     HIPCHK(pl->dtiles.ensure(tiles.size() * sizeof(uint2)));
     HIPCHK(h2d(pl->dtiles.p, tiles.data(), tiles.size() * sizeof(uint2), pl->xs));
     HIPCHK(pl->dflags.ensure(nsets * sizeof(unsigned)));
@@ -798,17 +825,18 @@ atmo_status build_plan(atmo_engine *e, const atmo_params *sets, int nsets, const
         maxNU = std::max(maxNU, sets[i].RES_NU);
         maxSW = std::max(maxSW, sets[i].SKY_W);
     }
-    pl->chJ1 = chunk_inscatterS(o.spherical_samples, maxNU, maxSW, true);
-    pl->chJ = chunk_inscatterS(o.spherical_samples, maxNU, maxSW, false);
+    auto jx_of_ch = [&](int ch) { return merged_extra(sets, nsets, o.spherical_samples, ch); };
+    pl->chJ1 = chunk_inscatterS(o.spherical_samples, maxNU, maxSW, jx_of_ch, true);
+    pl->chJ = chunk_inscatterS(o.spherical_samples, maxNU, maxSW, jx_of_ch, false);
+    pl->jx = pl->chJ ? jx_of_ch(pl->chJ) : 0;
     pl->chE1 = chunk_irradianceN(o.irradiance_samples, maxNU, true);
     pl->chEN = chunk_irradianceN(o.irradiance_samples, maxNU, false);
     if (!pl->chJ1 || !pl->chJ || !pl->chE1 || !pl->chEN)
         return fail(ATMO_ERR_UNSUPPORTED, "SKY_W %d: the ground irradiance row does not fit in LDS", maxSW);
-    pl->ldsJ1 = lds_inscatterS(pl->chJ1, maxNU, maxSW, true);
-    pl->ldsJ = lds_inscatterS(pl->chJ, maxNU, maxSW, false);
+    pl->ldsJ1 = lds_inscatterS(pl->chJ1, maxNU, maxSW, 0, true);
+    pl->ldsJ = lds_inscatterS(pl->chJ, maxNU, maxSW, pl->jx, false);
     pl->ldsE1 = lds_irradianceN(pl->chE1, maxNU, true);
     pl->ldsEN = lds_irradianceN(pl->chEN, maxNU, false);
-    (void)e;
     out = std::move(pl);
     return ATMO_OK;
 }
@@ -871,6 +899,21 @@ atmo_status check_out(const atmo_tables &t, int i) {
         return fail(ATMO_ERR_INVALID_ARGUMENT, "inscatter output of set %d is not 16-byte aligned", i);
     if (((uintptr_t)t.transmittance & 3) || ((uintptr_t)t.irradiance & 3))
         return fail(ATMO_ERR_INVALID_ARGUMENT, "outputs of set %d are not 4-byte aligned", i);
+    return ATMO_OK;
+}
+
+// ATMO_FLAG_CHECK_NUMERICS: counts, per set, the texels of the finished tables
+// that are non-finite or out of range (k_check) and reports the first bad set.
+atmo_status check_numerics(Plan &pl, hipStream_t hs) {
+    const int nsets = (int)pl.params.size();
+    HIPCHK(hipMemsetAsync(pl.dflags.p, 0, nsets * sizeof(unsigned), hs));
+    HIPCHK(launch_check((const SetDesc *)pl.dsets.p, (const uint2 *)pl.dtiles.p + pl.off[P_CHECK],
+                        pl.cnt[P_CHECK], (unsigned *)pl.dflags.p, hs));
+    std::vector<unsigned> flags(nsets);
+    HIPCHK(hipMemcpyAsync(flags.data(), pl.dflags.p, nsets * sizeof(unsigned), hipMemcpyDeviceToHost, hs));
+    HIPCHK(hipStreamSynchronize(hs));
+    for (int i = 0; i < nsets; ++i)
+        if (flags[i]) return fail(ATMO_ERR_NUMERIC, "set %d: %u texels non-finite or out of range", i, flags[i]);
     return ATMO_OK;
 }
 
@@ -988,7 +1031,7 @@ atmo_status atmo_engine_precompute(atmo_engine *e, const atmo_params *sets, int 
     if (!reuse) {
         HIPCHK(hipStreamSynchronize(hs));
         e->plan.reset();
-        if ((st = build_plan(e, sets, nsets, o, e->plan)) != ATMO_OK) return st;
+        if ((st = build_plan(e->cap, sets, nsets, o, e->plan)) != ATMO_OK) return st;
     }
     Plan &pl = *e->plan;
     bool changed = !reuse;
@@ -1044,17 +1087,8 @@ atmo_status atmo_engine_precompute(atmo_engine *e, const atmo_params *sets, int 
         // the tables of this call from the variants every later call runs
         if ((st = run_plan(e, pl, o, hs)) != ATMO_OK) return st;
     }
-    if (o.flags & ATMO_FLAG_CHECK_NUMERICS) {
-        HIPCHK(hipMemsetAsync(pl.dflags.p, 0, nsets * sizeof(unsigned), hs));
-        HIPCHK(launch_check((const SetDesc *)pl.dsets.p, (const uint2 *)pl.dtiles.p + pl.off[P_CHECK],
-                            pl.cnt[P_CHECK], (unsigned *)pl.dflags.p, hs));
-        std::vector<unsigned> flags(nsets);
-        HIPCHK(hipMemcpyAsync(flags.data(), pl.dflags.p, nsets * sizeof(unsigned), hipMemcpyDeviceToHost, hs));
-        HIPCHK(hipStreamSynchronize(hs));
-        for (int i = 0; i < nsets; ++i)
-            if (flags[i])
-                return fail(ATMO_ERR_NUMERIC, "set %d: %u texels non-finite or out of range", i, flags[i]);
-    }
+    if (o.flags & ATMO_FLAG_CHECK_NUMERICS)
+        if ((st = check_numerics(pl, hs)) != ATMO_OK) return st;
     g_err.clear();
     return ATMO_OK;
 }
@@ -1134,10 +1168,12 @@ static atmo_status pass_run(atmo_engine *e, const atmo_params *p, const atmo_opt
     case P_E1: HIPCHK(launch_irradiance1(S, T + off[P_E1], cnt[P_E1], hs)); break;
     case P_S1: HIPCHK(launch_inscatter1(S, T + off[P_S1], cnt[P_S1], o.inscatter_samples, lin_t(o), hs)); break;
     case P_J: {
-        const int ch = chunk_inscatterS(o.spherical_samples, p->RES_NU, p->SKY_W, first);
+        auto jx_of_ch = [&](int c) { return merged_extra(p, 1, o.spherical_samples, c); };
+        const int ch = chunk_inscatterS(o.spherical_samples, p->RES_NU, p->SKY_W, jx_of_ch, first);
         if (!ch) return fail(ATMO_ERR_UNSUPPORTED, "SKY_W %d: the ground irradiance row does not fit in LDS", p->SKY_W);
-        const size_t lds = lds_inscatterS(ch, p->RES_NU, p->SKY_W, first);
-        HIPCHK(launch_inscatterS(S, T + off[P_J], cnt[P_J], o.spherical_samples, ch, d + oJT, d + oJW, first,
+        const int jx = first ? 0 : jx_of_ch(ch);
+        const size_t lds = lds_inscatterS(ch, p->RES_NU, p->SKY_W, jx, first);
+        HIPCHK(launch_inscatterS(S, T + off[P_J], cnt[P_J], o.spherical_samples, ch, jx, d + oJT, d + oJW, first,
                                  lds, o.texels_per_thread ? o.texels_per_thread : jg_for_set(*p), hs));
         break;
     }

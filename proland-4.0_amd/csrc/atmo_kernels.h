@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
+
 #include "atmo_device.h"
 
 namespace atmo {
@@ -21,9 +23,14 @@ struct DirEHost {
 // kSphLdsBudget, the dynamic LDS a launch gets without raising the kernel's
 // limit; 0 = not even one row fits.  lds_* are the bytes for CH rows.
 constexpr size_t kSphLdsBudget = 64 * 1024;
-size_t lds_inscatterS(int CH, int RNU, int SW, bool first);
+// JX: extra LDS entries of the merged ground rows of one staging round of the
+// orders >= 3 pass (the sum over its theta rows of merged_row_extra, maximum
+// over the layers and sets of a launch)
+size_t lds_inscatterS(int CH, int RNU, int SW, int JX, bool first);
+// bound on the extra entries of theta row `it` (NS rows) of a layer at radius r
+int merged_row_extra(double r, double Rg, double Rt, int SW, int NS, int it);
 size_t lds_irradianceN(int CH, int RNU, bool first);
-int chunk_inscatterS(int NS, int RNU, int SW, bool first);
+int chunk_inscatterS(int NS, int RNU, int SW, const std::function<int(int)> &jx_of_ch, bool first);
 int chunk_irradianceN(int NI, int RNU, bool first);
 
 hipError_t launch_transmittance(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
@@ -37,7 +44,7 @@ hipError_t launch_tground(const SetDesc *sets, const uint2 *tiles, int ntiles, i
 hipError_t launch_inscatter1(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
                              bool linT, hipStream_t st);
 // G: texels per thread (2, 3, 4 or 8); CH: theta rows per staging round
-hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS, int CH,
+hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS, int CH, int JX,
                              const void *dirT, const void *dirW, bool first, size_t lds, int G,
                              hipStream_t st);
 hipError_t launch_irradianceN(const SetDesc *sets, const uint2 *tiles, int ntiles, int NI, int CH,

@@ -5,6 +5,10 @@
 // so sets of different resolutions share one launch.
 #include "atmo_kernels.h"
 
+#include <algorithm>
+#include <cmath>
+#include <type_traits>
+
 namespace atmo {
 
 // ---------------------------------------------------------------------------
@@ -242,7 +246,9 @@ __global__ __launch_bounds__(256) void k_inscatter1(const SetDesc *__restrict__ 
                     if (i1 == i0 + 1) {   // between two columns (not clamped to an edge)
                         const double x = (mus + 0.15) * s.tMu, x0 = xk[i0];
                         const float d = (float)(x - x0) * __builtin_amdgcn_rcpf(fmaf((float)x, (float)x0, 1.0f));
-                        a = atanf(d) * sc;
+                        // the FP64 coordinate can sit a rounding outside the FP32
+                        // cell: keep the filter weight in [0, 1] (a convex lerp)
+                        a = __builtin_amdgcn_fmed3f(atanf(d) * sc, 0.0f, 1.0f);
                     }
                 } else if (LINT) {   // (mu_s_i + 0.15) / 1.15: FP64 is cheap here
                     double ad;
@@ -333,6 +339,10 @@ __device__ __forceinline__ void stage_slice(const SetDesc &s, const float4 *cons
 // Entries of padding in front of the staged ground-irradiance row: texel
 // coordinates down to -(0.8/1.2) SW - 0.5 (mu_s = -1) plus rounding.
 __host__ __device__ __forceinline__ int e_pad(int SW) { return (2 * SW) / 3 + 3; }
+// The lookup positions pos = (nu1 + 1)/2 (RNU - 1) + 1 of a direction lie in
+// [1, RNU] up to FP32 rounding (|nu1| <= 1 + a few ulp); the merged ground rows
+// cover [kPosLo, RNU + kPosHi].
+constexpr float kPosLo = 1.0f - 1.0f / 256.0f, kPosHi = 1.0f / 256.0f;
 
 struct DirJ {      // per-theta constants of the NS x 2NS grid
     float ct, st, dw, pad;
@@ -346,7 +356,7 @@ struct DirJ {      // per-theta constants of the NS x 2NS grid
 template <bool FIRST, int G, bool CHUNK>
 __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ sets,
                                                     const uint2 *__restrict__ tiles, int NS, int CH,
-                                                    const DirJ *__restrict__ dirT,
+                                                    int JX, const DirJ *__restrict__ dirT,
                                                     const float2 *__restrict__ dirW) {
     constexpr int NT = FIRST ? 2 : 1;
     constexpr int PW = FIRST ? 3 : 2;      // float4 per staged pair entry
@@ -358,11 +368,16 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
     const int NU = s.RNU;
     const int NP = NU + 1;                 // pairs per theta
     const int NV = NU + 2;                 // slice values per theta
-    float4 *pairs = lds;                                   // [CH][NP][PW]
-    float4 *vtmp = pairs + (size_t)CH * NP * PW;           // [CH][NV][NT]
     const int EPAD = e_pad(s.SW);
-    float4 *epair = vtmp + (size_t)CH * NV * NT;           // [EPAD + SW + 1][2]
-    float4 *gdat = epair + (size_t)(EPAD + s.SW + 1) * 2;  // [CH][2]
+    // orders >= 3: a ground row holds NP + (its E cells - 1) merged entries
+    // (sky + ground terms); JX bounds the extra entries of a chunk's rows
+    const int JXr = FIRST ? 0 : JX;
+    const size_t npairs = (size_t)CH * NP + (size_t)JXr;
+    __shared__ int s_jxmax;
+    float4 *pairs = lds;                                   // rows of NP (+ JX) entries of PW
+    float4 *vtmp = pairs + npairs * PW;                    // [CH][NV][NT]
+    float4 *epair = vtmp + (size_t)CH * NV * NT;           // [EPAD + SW + 1][2] (order 2)
+    float4 *gdat = epair + (FIRST ? (size_t)(EPAD + s.SW + 1) * 2 : 0);   // [CH][3]
 
     const LayerGeom L = s.layers[l];
     const double r = fmin(fmax(L.r, s.Rg), s.Rt);          // inscatterS.glsl:112
@@ -372,12 +387,12 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
     const float4 *tabs[2] = {s.dSR4, s.dSM4};
     const double posAd = 0.5 * (NU - 1), posBd = posAd + 1.0;
 
-    // (2) row 0 of dE (irradiance at r = Rg reads only row 0 under CLAMP_TO_EDGE),
-    // with EPAD copies of texel 0 in front and one of texel SW-1 behind (zero
-    // differences): every texel coordinate the ground term can produce,
-    // (mu_s + 0.2)/1.2 SW - 0.5 for |mu_s| <= 1, then lands on an entry that
-    // reproduces the clamp, and the lookup needs no clamp instruction
-    for (int k = threadIdx.x; k < EPAD + s.SW + 1; k += blockDim.x) {
+    // (2) order 2: row 0 of dE (irradiance at r = Rg reads only row 0 under
+    // CLAMP_TO_EDGE), with EPAD copies of texel 0 in front and one of texel
+    // SW-1 behind (zero differences): every texel coordinate the ground term can
+    // produce, (mu_s + 0.2)/1.2 SW - 0.5 for |mu_s| <= 1, then lands on an entry
+    // that reproduces the clamp, and the lookup needs no clamp instruction
+    for (int k = threadIdx.x; FIRST && k < EPAD + s.SW + 1; k += blockDim.x) {
         const int k0 = min(max(k - EPAD, 0), s.SW - 1), k1 = min(max(k - EPAD + 1, 0), s.SW - 1);
         const float4 a = s.dE4[k0];
         const float4 b = s.dE4[k1];
@@ -424,6 +439,7 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
             const double cthetamin = -sqrt(1.0 - q * q);
             float4 g0 = make_float4(0.f, 0.f, 0.f, 0.f);
             double c1 = 0.0, ce0 = 0.0;
+            int jlo = 0, jhi = 0;
             if (ct < cthetamin) {
                 const double dg = -r * ct - sqrt(r * r * (ct * ct - 1.0) + s.Rg2);
                 const double mug = -(r * ct + dg) / s.Rg;
@@ -442,18 +458,54 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
                 const double e1 = dg / s.Rg * sc, e0 = (r * muS / s.Rg + 0.2) * sc - 0.5 + EPAD;
                 c1 = e1 / posAd;
                 ce0 = e0 - e1 * posBd / posAd;
+                // E cells the lookups of this row can reach: positions [1, NU] up to rounding
+                const float fc1 = (float)c1, fce0 = (float)ce0;
+                jlo = (int)__float2uint_rz(fmaf(kPosLo, fc1, fce0));
+                jhi = (int)__float2uint_rz(fmaf((float)NU + kPosHi, fc1, fce0));
+                // the lookups move xg by at most c1 (RNU - 1 + 2/256) <= 2.0158 e1
+                // (RNU >= 2): at most that many + 1 cells; the host sizes the rows
+                // with ceil(2.05 e1) + 2 per ground row (merged_extra)
+                jhi = min(jhi, jlo + (int)ceil(2.05 * e1) + 1);
             }
-            gdat[2 * itl] = g0;
-            gdat[2 * itl + 1] = make_float4((float)c1, (float)ce0, (float)(posAd * muS * ct + posBd),
+            gdat[3 * itl] = g0;
+            gdat[3 * itl + 1] = make_float4((float)c1, (float)ce0, (float)(posAd * muS * ct + posBd),
                                             (float)((kPI / NS) * (kPI / NS) * sin((it + 0.5) * (kPI / NS))));
+            gdat[3 * itl + 2] = make_float4(__int_as_float(jlo), __int_as_float(jhi),
+                                            __int_as_float(FIRST ? NP : NP + (jhi - jlo)), 0.f);
         }
         __syncthreads();
+        // first entry of every row: the sum of the entries of the rows before it
+        // (a ground row needs NP + jhi - jlo), and the widest ground row
+        if (threadIdx.x == 0) s_jxmax = 0;
+        for (int itl = threadIdx.x; itl < nth; itl += blockDim.x) {
+            int off = 0;
+            for (int i = 0; i < itl; ++i) off += __float_as_int(gdat[3 * i + 2].z);
+            gdat[3 * itl + 2].w = __int_as_float(off);
+        }
+        __syncthreads();
+        if (!FIRST)
+            for (int itl = threadIdx.x; itl < nth; itl += blockDim.x)
+                atomicMax(&s_jxmax, __float_as_int(gdat[3 * itl + 2].z) - NP);
+        // ground rows are a suffix of the chunk (ct decreases with theta): the
+        // first one, found by bisection on the flags
+        int ig0 = nth;
+        if (!FIRST) {
+            int lo = 0, hi = nth;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (gdat[3 * mid].w != 0.0f) hi = mid; else lo = mid + 1;
+            }
+            ig0 = lo;
+        }
+        auto rowoff = [&](int itl) { return __float_as_int(gdat[3 * itl + 2].w); };
+        __syncthreads();   // s_jxmax
         // (4) pairs {V[kk-1], V[kk] - V[kk-1]}
         for (int e = threadIdx.x; e < nth * NP; e += blockDim.x) {
             const int it = e / NP, kk = e % NP;
+            if (!FIRST && it >= ig0) continue;   // merged ground row (4b)
             const float4 a = vtmp[((size_t)it * NV + kk) * NT];
             const float4 b = vtmp[((size_t)it * NV + kk + 1) * NT];
-            float4 *q = pairs + (size_t)e * PW;
+            float4 *q = pairs + ((size_t)rowoff(it) + kk) * PW;
             if (!FIRST) {
                 // (V.rgb, D.r), (D.g, D.b): one 16-byte and one 8-byte LDS read
                 q[0] = make_float4(a.x, a.y, a.z, b.x - a.x);
@@ -465,6 +517,51 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
                 q[0] = make_float4(a.x, a.y, a.z, b.x - a.x);
                 q[1] = make_float4(b.y - a.y, b.z - a.z, am.x, am.y);
                 q[2] = make_float4(am.z, bm.x - am.x, bm.y - am.y, bm.z - am.z);
+            }
+        }
+        // (4b) orders >= 3, ground rows: the lookup of a ground direction is
+        // S(pos) + g0 E(xg(pos)) with xg = c1 pos + ce0, two piecewise-linear
+        // functions of the same position, with knots at the integers (slices) and
+        // where xg crosses an integer (E texels).  The merged row holds one entry
+        // per segment between consecutive knots of either kind: the value P at
+        // the segment's start p0 and the slope Q, so that a lookup is
+        // P + (pos - p0) Q.  The segment of (kk, jg) = (floor(pos), floor(xg)) is
+        // entry kk + jg - jlo (each knot raises one of the two by one); segments
+        // are found by intersecting every slice cell with every E cell of the row
+        // (FP64), so entries of zero-length segments at coinciding knots exist too.
+        if (!FIRST && ig0 < nth) {
+            const int nj = s_jxmax + 1, per_row = NP * nj;
+            for (int e = threadIdx.x; e < (nth - ig0) * per_row; e += blockDim.x) {
+                const int itl = ig0 + e / per_row, kk = (e % per_row) / nj, jj = e % nj;
+                // the whole merged row must lie inside the staging (guaranteed by
+                // the host's bound; a row that would not fit is left unwritten)
+                if (rowoff(itl) + __float_as_int(gdat[3 * itl + 2].z) > (int)npairs) continue;
+                const float4 g0 = gdat[3 * itl], g1 = gdat[3 * itl + 1], g2 = gdat[3 * itl + 2];
+                const int jlo = __float_as_int(g2.x), jhi = __float_as_int(g2.y), jg = jlo + jj;
+                if (jg > jhi) continue;
+                const double c1 = g1.x, ce0 = g1.y;
+                double lo = kk == 0 ? (double)kPosLo : (double)kk;
+                double hi = kk == NU ? (double)NU + kPosHi : kk + 1.0;
+                if (c1 > 0.0) {
+                    lo = fmax(lo, (jg - ce0) / c1);
+                    hi = fmin(hi, (jg + 1 - ce0) / c1);
+                }
+                if (lo > hi) continue;
+                lo = (double)(float)lo;   // P is the value at the stored (FP32) start
+                const float4 a = vtmp[((size_t)itl * NV + kk) * NT];
+                const float4 b = vtmp[((size_t)itl * NV + kk + 1) * NT];
+                const float4 ea = s.dE4[min(max(jg - EPAD, 0), s.SW - 1)];
+                const float4 eb = s.dE4[min(max(jg + 1 - EPAD, 0), s.SW - 1)];
+                const double u = lo - kk, v = lo * c1 + ce0 - jg;   // fractions at p0
+                const double dx = (double)b.x - a.x, dy = (double)b.y - a.y, dz = (double)b.z - a.z;
+                const double fx = (double)eb.x - ea.x, fy = (double)eb.y - ea.y, fz = (double)eb.z - ea.z;
+                const double Px = a.x + u * dx + g0.x * (ea.x + v * fx);
+                const double Py = a.y + u * dy + g0.y * (ea.y + v * fy);
+                const double Pz = a.z + u * dz + g0.z * (ea.z + v * fz);
+                const double Qx = dx + g0.x * c1 * fx, Qy = dy + g0.y * c1 * fy, Qz = dz + g0.z * c1 * fz;
+                float4 *q = pairs + ((size_t)rowoff(itl) + kk + jj) * PW;
+                q[0] = make_float4((float)Px, (float)Py, (float)Pz, (float)Qx);
+                q[1] = make_float4((float)Qy, (float)Qz, (float)lo, 0.f);
             }
         }
         __syncthreads();
@@ -542,6 +639,18 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
                     out[2] = fmaf(m2, z1, fmaf(f, b.y, a.z));
                 }
             };
+            // orders >= 3, ground direction: the merged sky + ground entry of the
+            // segment holding pos (4b)
+            auto glookup = [&](unsigned gb, float pos, float cg1, float cg0, float out[3]) {
+                const float xg = fmaf(pos, cg1, cg0);   // >= 0: EPAD is folded into cg0
+                const unsigned e = __float2uint_rz(pos) + __float2uint_rz(xg);
+                const float4 *pp = pairs + (gb + e * PW);
+                const float4 a = pp[0], b = pp[1];
+                const float f = pos - b.z;
+                out[0] = fmaf(f, a.w, a.x);
+                out[1] = fmaf(f, b.x, a.y);
+                out[2] = fmaf(f, b.y, a.z);
+            };
             // irradiance at the ground point seen in direction w, row 0 of dE
             auto ground_e = [&](float pos, float cg1, float cg0, float out[3]) {
                 const float xg = fmaf(pos, cg1, cg0);   // >= 0: EPAD is folded into cg0
@@ -557,67 +666,89 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
             for (int itl = 0; itl < nth; ++itl) {
                 const int it = c0 + itl;
                 const float ct = dirT[it].ct;
-                const float4 g0 = gdat[2 * itl], g1 = gdat[2 * itl + 1];
+                const float4 g0 = gdat[3 * itl], g1 = gdat[3 * itl + 1];
                 const bool ground = g0.w != 0.0f;
                 const float vzwz = vz * ct;
                 const float cx0 = fmaf(m2g, vzwz, g2p1);
                 const float dw = g1.w, cp = g1.z;
-                const float4 *pt = pairs + (size_t)itl * NP * PW;
+                const float4 *pt = pairs + (size_t)rowoff(itl) * PW;
+                // merged ground row: entry (floor(pos) + floor(xg)) - jlo, as an
+                // index that wraps modulo 2^32 (jlo may exceed the row offset)
+                const unsigned gbase = (unsigned)(rowoff(itl) - __float_as_int(gdat[3 * itl + 2].x)) * PW;
                 const float2 *wrow = dirW + (size_t)it * 2 * NS;
-                float2 wn = wrow[0];
-                // directions phi and 2 pi - phi together: same w.x, opposite w.y,
-                // hence the same nu2 = v.w (v has no y component) and weights
-                for (int ip = 0; ip < NS; ++ip) {
-                    const float2 w = wn;
-                    if (ip + 1 < NS) wn = wrow[ip + 1];  // scalar prefetch of the next direction
-                    const float nu2 = fmaf(vx, w.x, vzwz);
-                    const float q2 = fmaf(nu2 * dw, nu2, dw);
-                    const float y2 = __builtin_amdgcn_rsqf(fmaf(cx1, w.x, cx0));
-                    const float t2 = q2 * (y2 * y2 * y2);
-                    // betaR e^(-h/HR) pR(nu2) dw + betaMSca e^(-h/HM) pM(nu2) dw (:182-185)
-                    const float W0 = fmaf(A0, q2, B0 * t2), W1 = fmaf(A1, q2, B1 * t2),
-                                W2 = fmaf(A2, q2, B2 * t2);
-                    float pa[G], pb[G], lv[G][3];
-    #pragma unroll
-                    for (int g = 0; g < G; ++g) {
-                        const float base = fmaf(sxp[g], w.x, cp);
-                        pa[g] = fmaf(syp[g], w.y, base);
-                        pb[g] = fmaf(-syp[g], w.y, base);
-                        float la[3], lb[3], qa, qb;
-                        lookup(pt, pa[g], la, qa);
-                        lookup(pt, pb[g], lb, qb);
-                        if (FIRST) {
-    #pragma unroll
-                            for (int c = 0; c < 3; ++c) lv[g][c] = fmaf(qa, la[c], qb * lb[c]);
-                        } else {
-    #pragma unroll
-                            for (int c = 0; c < 3; ++c) lv[g][c] = la[c] + lb[c];
-                        }
-                    }
-                    if (ground) {
+                // the directions of one theta row; GR: a merged ground row of orders
+                // >= 3 (a compile-time branch, so the lookups of all G texels stay
+                // one straight sequence)
+                auto phi_row = [&](auto gr) {
+                    constexpr bool GR = decltype(gr)::value;
+                    float2 wn = wrow[0];
+                    // directions phi and 2 pi - phi together: same w.x, opposite w.y,
+                    // hence the same nu2 = v.w (v has no y component) and weights
+                    for (int ip = 0; ip < NS; ++ip) {
+                        const float2 w = wn;
+                        if (ip + 1 < NS) wn = wrow[ip + 1];  // scalar prefetch of the next direction
+                        const float nu2 = fmaf(vx, w.x, vzwz);
+                        const float q2 = fmaf(nu2 * dw, nu2, dw);
+                        const float y2 = __builtin_amdgcn_rsqf(fmaf(cx1, w.x, cx0));
+                        const float t2 = q2 * (y2 * y2 * y2);
+                        // betaR e^(-h/HR) pR(nu2) dw + betaMSca e^(-h/HM) pM(nu2) dw (:182-185)
+                        const float W0 = fmaf(A0, q2, B0 * t2), W1 = fmaf(A1, q2, B1 * t2),
+                                    W2 = fmaf(A2, q2, B2 * t2);
+                        float pa[G], pb[G], lv[G][3];
     #pragma unroll
                         for (int g = 0; g < G; ++g) {
-                            float ea[3], eb[3];
-                            ground_e(pa[g], g1.x, g1.y, ea);
-                            ground_e(pb[g], g1.x, g1.y, eb);
-                            lv[g][0] = fmaf(g0.x, ea[0] + eb[0], lv[g][0]);
-                            lv[g][1] = fmaf(g0.y, ea[1] + eb[1], lv[g][1]);
-                            lv[g][2] = fmaf(g0.z, ea[2] + eb[2], lv[g][2]);
+                            const float base = fmaf(sxp[g], w.x, cp);
+                            pa[g] = fmaf(syp[g], w.y, base);
+                            pb[g] = fmaf(-syp[g], w.y, base);
+                            float la[3], lb[3], qa, qb;
+                            if constexpr (!FIRST && GR) {
+                                glookup(gbase, pa[g], g1.x, g1.y, la);
+                                glookup(gbase, pb[g], g1.x, g1.y, lb);
+                            } else {
+                                lookup(pt, pa[g], la, qa);
+                                lookup(pt, pb[g], lb, qb);
+                            }
+                            if (FIRST) {
+    #pragma unroll
+                                for (int c = 0; c < 3; ++c) lv[g][c] = fmaf(qa, la[c], qb * lb[c]);
+                            } else {
+    #pragma unroll
+                                for (int c = 0; c < 3; ++c) lv[g][c] = la[c] + lb[c];
+                            }
+                        }
+                        if (FIRST && ground) {   // order 2: the separate ground term (uniform branch)
+    #pragma unroll
+                            for (int g = 0; g < G; ++g) {
+                                float ea[3], eb[3];
+                                ground_e(pa[g], g1.x, g1.y, ea);
+                                ground_e(pb[g], g1.x, g1.y, eb);
+                                lv[g][0] = fmaf(g0.x, ea[0] + eb[0], lv[g][0]);
+                                lv[g][1] = fmaf(g0.y, ea[1] + eb[1], lv[g][1]);
+                                lv[g][2] = fmaf(g0.z, ea[2] + eb[2], lv[g][2]);
+                            }
+                        }
+    #pragma unroll
+                        for (int g = 0; g < G; ++g) {
+                            acc[g][0] = fmaf(lv[g][0], W0, acc[g][0]);
+                            acc[g][1] = fmaf(lv[g][1], W1, acc[g][1]);
+                            acc[g][2] = fmaf(lv[g][2], W2, acc[g][2]);
                         }
                     }
-    #pragma unroll
-                    for (int g = 0; g < G; ++g) {
-                        acc[g][0] = fmaf(lv[g][0], W0, acc[g][0]);
-                        acc[g][1] = fmaf(lv[g][1], W1, acc[g][1]);
-                        acc[g][2] = fmaf(lv[g][2], W2, acc[g][2]);
-                    }
-                }
+                };
+                if (!FIRST && ground) phi_row(std::true_type{});
+                else phi_row(std::false_type{});
             }
     #pragma unroll
             for (int g = 0; g < G; ++g) {
                 const int k = kb + g;
                 if (k < NU) {
                     const size_t o = ((size_t)l * s.RMU + y) * s.W4 + (size_t)k * s.RMUS + m;
+                    if (!FIRST && last) {
+                        // dJ is a sum of non-negative terms; a merged ground entry
+                        // (FP32 P and Q) can undershoot a zero value by rounding
+                        acc[g][0] = fmaxf(acc[g][0], 0.f); acc[g][1] = fmaxf(acc[g][1], 0.f);
+                        acc[g][2] = fmaxf(acc[g][2], 0.f);
+                    }
                     if (s.hs && last) {
                         acc[g][0] = q16(acc[g][0]); acc[g][1] = q16(acc[g][1]); acc[g][2] = q16(acc[g][2]);
                     }
@@ -785,6 +916,19 @@ struct SnEntry {
     float4 ir;    // 1 / r_i, r_i - r
 };
 
+// The (r, mu) blend of one table column x: sum_q w_q dJ[row_q][x], in one fixed
+// FP32 order.  Both forms of k_inscatterN (direct gathers, LDS-staged rows)
+// blend with this, then lerp two columns in x as fma(ax, b1 - b0, b0), so
+// they give the same bits.
+__device__ __forceinline__ float3 sn_blend(const float4 &w, float3 t0, float3 t1, float3 t2, float3 t3) {
+    return make_float3(fmaf(w.w, t3.x, fmaf(w.z, t2.x, fmaf(w.y, t1.x, w.x * t0.x))),
+                       fmaf(w.w, t3.y, fmaf(w.z, t2.y, fmaf(w.y, t1.y, w.x * t0.y))),
+                       fmaf(w.w, t3.z, fmaf(w.z, t2.z, fmaf(w.y, t1.z, w.x * t0.z))));
+}
+__device__ __forceinline__ float3 sn_lerp(float ax, float3 b0, float3 b1) {
+    return make_float3(fmaf(ax, b1.x - b0.x, b0.x), fmaf(ax, b1.y - b0.y, b0.y), fmaf(ax, b1.z - b0.z, b0.z));
+}
+
 // Texel offset of the sample's mu_s_i = (nu t + mu_s r) / r_i inside a nu
 // slice (texture4D's uMuS * RMUS - 0.5), evaluated relative to the texel's own
 // mu_s.  Near the start of a long, nearly horizontal ray mu_s_i moves away from
@@ -823,9 +967,10 @@ __device__ __forceinline__ float sn_txs(const SetDesc &s, float nu, float mus, f
 // 4 (r, mu) table rows of every row it covers into one row
 // B_i[x] = sum_q w_q dJ[row_q][x] in LDS (coalesced loads, 4 per texel
 // instead of 8, shared by all texels that hit x), then each texel reads two
-// neighbours of B_i and lerps in x.  Both compute the same trilinear filter;
-// they differ only in FP32 rounding order.  `sb` samples are blended per LDS
-// round (dynamic LDS: sb * rows * RNU*RMU_S float4).
+// neighbours of B_i and lerps in x.  The direct form blends the two columns
+// it needs in the same order (sn_blend, sn_lerp), so both give the same bits.
+// `sb` samples are blended per LDS round (dynamic LDS: sb * rows * RNU*RMU_S
+// float4).
 template <bool STAGED, bool LIN4>   // LIN4: linear inscatter parameterisation (per-sample mu_s)
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_inscatterN(const SetDesc *__restrict__ sets,
                                                     const uint2 *__restrict__ tiles,
@@ -951,36 +1096,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                     const unsigned is = min(__float2uint_rz(txs), ismax);
                     const float ax = txs - (float)is;
                     const unsigned i0 = kbase + is;
-                    float3 v = make_float3(0.f, 0.f, 0.f);
-                    const float wr[4] = {en.w.x, en.w.y, en.w.z, en.w.w};
-                    const unsigned br[4] = {(unsigned)en.base.x, (unsigned)en.base.y, (unsigned)en.base.z,
-                                            (unsigned)en.base.w};
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const unsigned off = br[q] + i0 * jb;
-                        const float3 c0v = ld3((const gfloat4 *)(J + off));
-                        const float3 c1v = ld3((const gfloat4 *)(J + off + jb));
-                        const float wq = wr[q], wa = wr[q] * ax;
-                        v.x = fmaf(wq, c0v.x, fmaf(wa, c1v.x - c0v.x, v.x));
-                        v.y = fmaf(wq, c0v.y, fmaf(wa, c1v.y - c0v.y, v.y));
-                        v.z = fmaf(wq, c0v.z, fmaf(wa, c1v.z - c0v.z, v.z));
-                    }
+                    // the blended columns i0 and i0 + 1 (as the staged form's rows hold them)
+                    auto column = [&](unsigned x) {
+                        const unsigned o = x * jb;
+                        return sn_blend(en.w, ld3((const gfloat4 *)(J + (unsigned)en.base.x + o)),
+                                        ld3((const gfloat4 *)(J + (unsigned)en.base.y + o)),
+                                        ld3((const gfloat4 *)(J + (unsigned)en.base.z + o)),
+                                        ld3((const gfloat4 *)(J + (unsigned)en.base.w + o)));
+                    };
+                    float3 v = sn_lerp(ax, column(i0), column(i0 + 1));
                     if (fk[j] != 0.0f) {  // off-grid nu: blend with the next slice (k0 + 1 <= RNU - 1)
                         unsigned j0 = i0 + (unsigned)s.RMUS;
                         float bx = ax;
                         if (j0 > W4m2) { j0 = W4m2; bx = 1.0f; }   // past the row's last column
-                        float3 v2 = make_float3(0.f, 0.f, 0.f);
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) {
-                            const unsigned off = br[q] + j0 * jb;
-                            const float3 c0v = ld3((const gfloat4 *)(J + off));
-                            const float3 c1v = ld3((const gfloat4 *)(J + off + jb));
-                            const float wq = wr[q], wa = wr[q] * bx;
-                            v2.x = fmaf(wq, c0v.x, fmaf(wa, c1v.x - c0v.x, v2.x));
-                            v2.y = fmaf(wq, c0v.y, fmaf(wa, c1v.y - c0v.y, v2.y));
-                            v2.z = fmaf(wq, c0v.z, fmaf(wa, c1v.z - c0v.z, v2.z));
-                        }
-                        v = lerp3(v, v2, fk[j]);
+                        v = lerp3(v, sn_lerp(bx, column(j0), column(j0 + 1)), fk[j]);
                     }
                     a0 = fmaf(v.x, en.tw.x, a0);
                     a1 = fmaf(v.y, en.tw.y, a1);
@@ -1001,16 +1130,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                         const int rowj = kq[j] >> 16;
                         const SnEntry &en = ent[rowj][s0 + k];
                         const unsigned xo = (unsigned)(e - rowj * s.W4) * jb;
-                        const float3 t0 = ld3((const gfloat4 *)(J + (unsigned)en.base.x + xo));
-                        const float3 t1 = ld3((const gfloat4 *)(J + (unsigned)en.base.y + xo));
-                        const float3 t2 = ld3((const gfloat4 *)(J + (unsigned)en.base.z + xo));
-                        const float3 t3 = ld3((const gfloat4 *)(J + (unsigned)en.base.w + xo));
-                        float4 b;
-                        b.x = fmaf(en.w.w, t3.x, fmaf(en.w.z, t2.x, fmaf(en.w.y, t1.x, en.w.x * t0.x)));
-                        b.y = fmaf(en.w.w, t3.y, fmaf(en.w.z, t2.y, fmaf(en.w.y, t1.y, en.w.x * t0.y)));
-                        b.z = fmaf(en.w.w, t3.z, fmaf(en.w.z, t2.z, fmaf(en.w.y, t1.z, en.w.x * t0.z)));
-                        b.w = 0.f;
-                        blend[(size_t)k * per + e] = b;
+                        const float3 b = sn_blend(en.w, ld3((const gfloat4 *)(J + (unsigned)en.base.x + xo)),
+                                                  ld3((const gfloat4 *)(J + (unsigned)en.base.y + xo)),
+                                                  ld3((const gfloat4 *)(J + (unsigned)en.base.z + xo)),
+                                                  ld3((const gfloat4 *)(J + (unsigned)en.base.w + xo)));
+                        blend[(size_t)k * per + e] = make_float4(b.x, b.y, b.z, 0.f);
                     }
                 }
                 __syncthreads();
@@ -1033,17 +1157,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                         const float ax = txs - (float)is;
                         const unsigned i0 = kbase + is;
                         const float4 *B = blend + (size_t)k * per + rowbase;
-                        const float4 p0 = B[i0], p1 = B[i0 + 1];
-                        float3 v = make_float3(fmaf(ax, p1.x - p0.x, p0.x), fmaf(ax, p1.y - p0.y, p0.y),
-                                               fmaf(ax, p1.z - p0.z, p0.z));
+                        float3 v = sn_lerp(ax, f3(B[i0]), f3(B[i0 + 1]));
                         if (fk[j] != 0.0f) {  // off-grid nu: blend with the next slice (k0 + 1 <= RNU - 1)
                             unsigned j0 = i0 + (unsigned)s.RMUS;
                             float bx = ax;
                             if (j0 > W4m2) { j0 = W4m2; bx = 1.0f; }   // past the row's last column
-                            const float4 q0 = B[j0], q1 = B[j0 + 1];
-                            const float3 v2 = make_float3(fmaf(bx, q1.x - q0.x, q0.x), fmaf(bx, q1.y - q0.y, q0.y),
-                                                          fmaf(bx, q1.z - q0.z, q0.z));
-                            v = lerp3(v, v2, fk[j]);
+                            v = lerp3(v, sn_lerp(bx, f3(B[j0]), f3(B[j0 + 1])), fk[j]);
                         }
                         a0 = fmaf(v.x, en.tw.x, a0);
                         a1 = fmaf(v.y, en.tw.y, a1);
@@ -1176,19 +1295,35 @@ __global__ __launch_bounds__(256) void k_check(const SetDesc *__restrict__ sets,
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-size_t lds_inscatterS(int CH, int RNU, int SW, bool first) {
+size_t lds_inscatterS(int CH, int RNU, int SW, int JX, bool first) {
     const int NT = first ? 2 : 1, PW = first ? 3 : 2;
-    return sizeof(float4) * ((size_t)CH * (RNU + 1) * PW + (size_t)CH * (RNU + 2) * NT +
-                             (size_t)(e_pad(SW) + SW + 1) * 2 + (size_t)CH * 2);
+    const size_t npairs = (size_t)CH * (RNU + 1) + (first ? 0 : (size_t)JX);
+    return sizeof(float4) * (npairs * PW + (size_t)CH * (RNU + 2) * NT +
+                             (first ? (size_t)(e_pad(SW) + SW + 1) * 2 : 0) + (size_t)CH * 3);
+}
+int merged_row_extra(double r, double Rg, double Rt, int SW, int NS, int it) {
+    // the kernel's r, clamped as inscatterS.glsl:112 does, and its ground test
+    r = std::min(std::max(r, Rg), Rt);
+    const double ct = std::cos((it + 0.5) * (kPI / NS));
+    if (ct >= 1e-9) return 0;   // theta < 90 deg: never a ground row
+    const double q = Rg / r, cthetamin = -std::sqrt(1.0 - q * q);
+    // ground distance, or (where the device's ground test could round either
+    // way) the tangent distance, which bounds the ground distance of every ray
+    // that hits the ground
+    const double dg = ct < cthetamin - 1e-9 ? -r * ct - std::sqrt(r * r * (ct * ct - 1.0) + Rg * Rg)
+                                            : std::sqrt(std::max(r * r - Rg * Rg, 0.0));
+    const double e1 = dg / Rg * (SW / 1.2);
+    // the kernel keeps at most ceil(2.05 e1) + 1 extra entries in a row
+    return (int)std::ceil(2.05 * e1) + 2;
 }
 size_t lds_irradianceN(int CH, int RNU, bool first) {
     const int NT = first ? 2 : 1;
     return sizeof(float4) * ((size_t)CH * (RNU + 1) * NT * 2 + (size_t)CH * (RNU + 2) * NT) +
            sizeof(float) * 3 * 16;
 }
-int chunk_inscatterS(int NS, int RNU, int SW, bool first) {
+int chunk_inscatterS(int NS, int RNU, int SW, const std::function<int(int)> &jx_of_ch, bool first) {
     for (int ch = NS; ch >= 1; --ch)
-        if (lds_inscatterS(ch, RNU, SW, first) <= kSphLdsBudget) return ch;
+        if (lds_inscatterS(ch, RNU, SW, first ? 0 : jx_of_ch(ch), first) <= kSphLdsBudget) return ch;
     return 0;
 }
 int chunk_irradianceN(int NI, int RNU, bool first) {
@@ -1228,14 +1363,14 @@ hipError_t launch_inscatter1(const SetDesc *sets, const uint2 *tiles, int ntiles
         hipLaunchKernelGGL(k_inscatter1<false>, dim3(ntiles), dim3(256), 0, st, sets, tiles, nsamp);
     return hipGetLastError();
 }
-hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS, int CH,
+hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS, int CH, int JX,
                              const void *dirT, const void *dirW, bool first, size_t lds, int G,
                              hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
-    if (CH < 1 || CH > NS) return hipErrorInvalidValue;
+    if (CH < 1 || CH > NS || JX < 0) return hipErrorInvalidValue;
 #define ATMO_LJ(F, GG, C)                                                                          \
     hipLaunchKernelGGL((k_inscatterS<F, GG, C>), dim3(ntiles), dim3(256), lds, st, sets, tiles, NS, \
-                       CH, (const DirJ *)dirT, (const float2 *)dirW)
+                       CH, JX, (const DirJ *)dirT, (const float2 *)dirW)
     if (G != 2 && G != 3 && G != 4 && G != 8) return hipErrorInvalidValue;
     if (CH < NS) {   // chunked staging: one variant (all G give the same bits)
         if (first) ATMO_LJ(true, 4, true); else ATMO_LJ(false, 4, true);

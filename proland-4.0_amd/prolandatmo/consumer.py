@@ -19,6 +19,21 @@ class AtmoView(ctypes.Structure):
 
 assert ctypes.sizeof(AtmoView) == 64
 
+
+class AtmoTextureDesc(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 64), ("source", ctypes.c_char * 512), ("format", ctypes.c_char * 16),
+                ("kind", ctypes.c_int), ("format_channels", ctypes.c_int), ("half", ctypes.c_int),
+                ("min_linear", ctypes.c_int), ("mag_linear", ctypes.c_int), ("clamp_to_edge", ctypes.c_int * 3),
+                ("width", ctypes.c_int), ("height", ctypes.c_int), ("depth", ctypes.c_int),
+                ("components", ctypes.c_int)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        for k in ("name", "source", "format"):
+            d[k] = d[k].decode()
+        d["clamp_to_edge"] = list(self.clamp_to_edge)
+        return d
+
 # every symbol include/atmo_consumer.h declares, with its signature
 SIGNATURES = {
     "atmo_sky_radiance": (ctypes.c_int, [_pp, _vp, _vp, ctypes.POINTER(AtmoView), _vp, ctypes.c_int,
@@ -26,6 +41,9 @@ SIGNATURES = {
     "atmo_in_scattering": (ctypes.c_int, [_pp, _vp, _vp, ctypes.POINTER(AtmoView), _vp, ctypes.c_int,
                                           _vp, _vp, _vp]),
     "atmo_sun_sky_irradiance": (ctypes.c_int, [_pp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
+    "atmo_texture_load_desc": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(AtmoTextureDesc)]),
+    "atmo_texture_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t,
+                                         ctypes.POINTER(AtmoTextureDesc)]),
 }
 
 _bound = False
@@ -96,3 +114,24 @@ def sun_sky_irradiance(p, T, E, rmus, stream=None):
     check(lib().atmo_sun_sky_irradiance(p, _ptr(T), _ptr(E), _ptr(a), n, _ptr(sun), _ptr(sky),
                                         _stream(stream)))
     return sun, sky
+
+
+def texture_desc(xml_path):
+    """atmo_texture_load_desc: the descriptor's attributes and its source's sizes."""
+    d = AtmoTextureDesc()
+    check(lib().atmo_texture_load_desc(xml_path.encode(), ctypes.byref(d)))
+    return d.as_dict()
+
+
+def load_texture(xml_path):
+    """atmo_texture_load: (payload as the GPU texture stores it, numpy float32 in
+    the .raw layout [depth][height][width][components] or [height][width][components],
+    descriptor dict)."""
+    import numpy as np
+    d = texture_desc(xml_path)
+    shape = ((d["depth"],) if d["depth"] > 0 else ()) + (d["height"], d["width"], d["components"])
+    a = np.zeros(int(np.prod(shape)), dtype=np.float32)
+    out = AtmoTextureDesc()
+    check(lib().atmo_texture_load(xml_path.encode(), a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                  a.size, ctypes.byref(out)))
+    return a.reshape(shape), out.as_dict()

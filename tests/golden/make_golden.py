@@ -18,6 +18,11 @@ come from the pinned FP64 oracle (oracle/, checked in tests/test_oracle_pins.py)
                       texels (a seeded subset of a seeded 2 M-texel sample).
                       About 2.5 h of the oracle on 7 threads: run it alone,
                       `python tests/golden/make_golden.py precision_hires`.
+  batch_classes.npz   one set of each inscatter resolution class of the 256-set
+                      batch (seed 0x5EED0003 stream: 16x64x16x4, 32x128x32x8,
+                      48x192x48x12; T 256x64, E 64x16, orders 4): T and E in full,
+                      S at 4096 seeded texels, with each set's index in the batch
+                      (`... make_golden.py batch_classes`, ~15 min on 7 threads)
   earth_ref_half.npz  the reference configuration with ATMO_FLAG_HALF_STORAGE (every
                       table rounded to FP16 as the reference's RGB16F / RGBA16F
                       textures): T and E in full, S at the 4096 texels of
@@ -127,6 +132,30 @@ def half_storage():
     line("E", E, E64)
     for c in range(4):
         line("S." + "rgba"[c], S[..., c], S64[..., c])
+
+
+def batch_classes():
+    """One set of each inscatter resolution class of the 256-set batch (the
+    first of its class in the seed 0x5EED0003 stream; T 256x64, E 64x16,
+    orders 4): T and E in full, S at 4096 seeded texels each."""
+    from prolandatmo.params import RES_CLASSES
+    sets, _ = draw_sets(SEED_BATCH, 256)
+    out = {}
+    for c, res in enumerate(RES_CLASSES):
+        i = next(k for k, q in enumerate(sets) if (q.RES_R, q.RES_MU, q.RES_MU_S, q.RES_NU) == res)
+        p = sets[i]
+        t = time.time()
+        T, E, S = O.precompute(p, options())
+        print("batch set %d (%dx%dx%dx%d) oracle: %.1f s on %d threads"
+              % ((i,) + res + (time.time() - t, O.get_threads())), flush=True)
+        flat = S.reshape(-1, 4)
+        idx = np.sort(np.random.default_rng(0x5EED0007 + c).choice(flat.shape[0], size=4096, replace=False))
+        out["index%d" % c] = np.int32(i)
+        out["params%d" % c] = pvec(p)
+        out["T%d" % c], out["E%d" % c] = T.astype(np.float32), E
+        out["S_idx%d" % c], out["S_val%d" % c] = idx.astype(np.int32), flat[idx]
+        out["S_max%d" % c] = np.abs(S).max(axis=(0, 1, 2))
+    np.savez_compressed(os.path.join(HERE, "batch_classes.npz"), **out)
 
 
 if __name__ == "__main__":

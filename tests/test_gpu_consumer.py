@@ -25,7 +25,7 @@ from oracle import oracle as O  # noqa: E402
 from prolandatmo import consumer as C  # noqa: E402
 from prolandatmo import lib as L  # noqa: E402
 from prolandatmo.metrics import summary  # noqa: E402
-from prolandatmo.params import SEED_BATCH, draw_sets, earth, with_res  # noqa: E402
+from prolandatmo.params import FLAG_NO_AUTOTUNE, SEED_BATCH, draw_sets, earth, options, with_res  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -155,7 +155,8 @@ def test_raw_files_render_like_the_in_memory_tables(eng, tmp_path):
     p = ATMOS[0]
     d = str(tmp_path)
     L.preprocess(p, d)
-    (dT, dE, dS), = eng.precompute(p)
+    # preprocessAtmo's one-off precompute runs the default kernel variants
+    (dT, dE, dS), = eng.precompute(p, options(flags=FLAG_NO_AUTOTUNE))
     torch.cuda.synchronize()
     t, _ = L.read_raw(os.path.join(d, "transmittance.raw"))
     s, _ = L.read_raw(os.path.join(d, "inscatter.raw"))
@@ -167,3 +168,68 @@ def test_raw_files_render_like_the_in_memory_tables(eng, tmp_path):
     b, eb = C.sky_radiance(p, fT, fS, v, dirs)
     torch.cuda.synchronize()
     assert torch.equal(a, b) and torch.equal(ea, eb)
+
+
+def _descriptor(d, name, kind, source):
+    """An ork texture descriptor of the form of src/demo/textures/atmo/*.xml:25."""
+    axes = ("wraps", "wrapt", "wrapr")[: (3 if kind == "texture3D" else 2)]
+    path = os.path.join(d, name + ".xml")
+    with open(path, "w") as f:
+        f.write('<?xml version="1.0" ?>\n<%s name="%s" source="%s" internalformat="RGBA16F" min="LINEAR" '
+                'mag="LINEAR" %s/>\n' % (kind, name, source, " ".join('%s="CLAMP_TO_EDGE"' % a for a in axes)))
+    return path
+
+
+def test_rgba16f_uploads_through_the_descriptors_render_like_the_oracle(eng, tmp_path):
+    """What the demo samples: the .raw files loaded through ork descriptors with
+    internalformat RGBA16F (every value an IEEE binary16 value), LINEAR and
+    CLAMP_TO_EDGE.  The GPU consumer on those tables against the oracle
+    consumer on the same tables quantised by NumPy's binary16 conversion (bar
+    as for the same tables above), and the quantised sky within 1e-2 of the
+    FP32 tables' sky (the FP16 storage error the reference's textures add)."""
+    p = ATMOS[0]
+    d = str(tmp_path)
+    L.preprocess(p, d, options(flags=FLAG_NO_AUTOTUNE))
+    xT = _descriptor(d, "transmittance", "texture2D", "transmittance.raw")
+    xE = _descriptor(d, "irradiance", "texture2D", "irradiance.raw")
+    xS = _descriptor(d, "inscatter", "texture3D", "inscatter.raw")
+    T16, dT = C.load_texture(xT)
+    E16, dE = C.load_texture(xE)
+    S16, dS = C.load_texture(xS)
+    assert (dT["width"], dT["height"], dT["components"]) == (p.TRANSMITTANCE_W, p.TRANSMITTANCE_H, 3)
+    assert (dS["width"], dS["height"], dS["depth"], dS["components"]) == (
+        p.RES_MU_S * p.RES_NU, p.RES_MU, p.RES_R, 4)
+    t32, _ = L.read_raw(os.path.join(d, "transmittance.raw"))
+    s32, _ = L.read_raw(os.path.join(d, "inscatter.raw"))
+    e32, _ = L.read_raw(os.path.join(d, "irradiance.raw"))
+    # the loader's quantisation is binary16's
+    for a, b in ((T16, t32), (E16, e32), (S16, s32)):
+        assert np.array_equal(a.ravel().view(np.uint32), b.astype(np.float16).astype(np.float32).view(np.uint32))
+    S16 = S16.reshape(p.RES_R, p.RES_MU, p.RES_NU * p.RES_MU_S, 4)
+    gT, gE, gS = _dev(T16), _dev(E16), _dev(S16)
+    fT, fS = _dev(t32.reshape(T16.shape)), _dev(s32.reshape(S16.shape))
+    T64, E64, S64 = (a.astype(np.float64) for a in (T16, E16, S16))
+    dirs = panorama()
+    for ci, cam in enumerate(cameras(p)):
+        for si, sun in enumerate(SUNS):
+            v = C.view(cam, sun)
+            rad, ext = C.sky_radiance(p, gT, gS, v, dirs)
+            orad, oext = O.sky_radiance(p, T64, S64, cam, dirs, sun)
+            _assert("16F sky c%d s%d" % (ci, si), rad[:, :3].double().cpu().numpy(), orad, 1e-4)
+            _assert("16F ext c%d s%d" % (ci, si), ext[:, :3].double().cpu().numpy(), oext, 1e-4)
+            frad, _ = C.sky_radiance(p, fT, fS, v, dirs)
+            _assert("16F vs 32F c%d s%d" % (ci, si), rad[:, :3].double().cpu().numpy(),
+                    frad[:, :3].double().cpu().numpy(), 1e-2)
+    cam = cameras(p)[1]
+    pts = (cam + panorama(12, 16) * 5.0).astype(np.float32).astype(np.float64)
+    v = C.view(cam, SUNS[0])
+    rad, ext = C.in_scattering(p, gT, gS, v, pts)
+    orad, oext = O.in_scattering(p, T64, S64, cam, pts, SUNS[0])
+    _assert("16F aerial", rad[:, :3].double().cpu().numpy(), orad, 1e-4, 1e-2)
+    r = np.linspace(float(p.Rg), float(p.Rt), 5)
+    m = np.linspace(-0.3, 1.0, 13)
+    rmus = np.stack(np.meshgrid(r, m, indexing="ij"), -1).reshape(-1, 2).astype(np.float32)
+    sun, sky = C.sun_sky_irradiance(p, gT, gE, rmus)
+    osun, osky = O.sun_sky_irradiance(p, T64, E64, rmus.astype(np.float64))
+    _assert("16F sun", sun[:, :3].double().cpu().numpy(), osun, 1e-5)
+    _assert("16F skyE", sky[:, :3].double().cpu().numpy(), osky, 1e-5)
