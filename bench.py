@@ -100,6 +100,44 @@ def cpu_baseline(p, o, budget_threads=16):
                        "extrapolated per precompute" % (int(scale), tE + t4, est))}
 
 
+def surrogate_block(steps, exact_orders=1):
+    """The learned part on its own workloads: training on 2048 seeded sets of the
+    batch distribution (32768 inscatter texels + every irradiance texel per
+    set, batch 65536, Adam), then 1024 unseen sets at 32x128x32x8 (4 orders):
+    plain batched tables/s, surrogate tables/s with the network trusted and
+    with the 1 % judge, the judge's verdict, and the per-set max relative
+    error of S against the plain tables (floored at 1e-4 of the table max)."""
+    import torch
+    from prolandatmo import surrogate_study as SS
+    torch.cuda.empty_cache()
+    a = SS.parser().parse_args([])
+    a.steps = steps
+    a.exact_orders = exact_orders
+    r = SS.run(a, log=log)
+    nj, pl = r["surrogate_no_judge"], r["plain"]
+    j, ja = r["surrogate_judged"], r["surrogate_judged_accepted"]
+    return {"workload": "train: 2048 sets (seed 0x5EED0004), 32768 texels + E per set, batch 65536, "
+                        "%d Adam steps; infer: 1024 unseen sets (seed 0x5EED0005) at 32x128x32x8, "
+                        "4 orders; orders 1-%d exact, the network predicts the rest" % (steps, exact_orders),
+            "training": {k: r["training"][k] for k in ("ms_per_step", "tflops_model", "loss_first",
+                                                       "loss_last")},
+            "dataset_s": r["dataset"]["s"],
+            "plain_tables_per_s": pl["tables_per_s"],
+            "surrogate_trusted_tables_per_s": nj["tables_per_s"],
+            "surrogate_trusted_speedup": nj["speedup_vs_plain"],
+            "surrogate_judged_tables_per_s": j["tables_per_s"],
+            "surrogate_judged_speedup": j["speedup_vs_plain"],
+            "judge_passing_tolerance": ja["tolerance"],
+            "surrogate_judged_speedup_at_passing_tolerance": ja["speedup_vs_plain"],
+            "judge": {"fraction_of_sets": a.judge, "tolerance": a.tol,
+                      "judged_sets": j["report"]["judged"], "accepted": j["report"]["accepted"],
+                      "fallback_sets": j["report"]["fallback_sets"],
+                      "judged_max_rel_err_S": j["report"]["max_rel_err_S"]},
+            "per_set_max_rel_err_S": nj["max_rel_err_S"],
+            "mean_rel_err_S": nj["mean_rel_err_S"],
+            "ms": {"exact": nj["report"]["ms_exact"], "mlp": nj["report"]["ms_mlp"]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +145,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the surrogate block (training on 2048 sets + inference on 1024)")
+    ap.add_argument("--sur-steps", type=int, default=3000)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also copy a 256 MiB device buffer (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -170,6 +211,13 @@ def main():
             "formulation_flop_per_launch": formulation,
             "formulation_tflops": round(formulation / (avg_ms * 1e-3) / 1e12, 2),
             "formulation_speedup": round(formulation / executed, 2) if executed else None}
+    valu = profiled(COUNTER_PROFILE, "valu_insts", kname)
+    if valu:
+        # VALU wave instructions per texel-direction (SQ_INSTS_VALU x 64 lanes over
+        # RES_R RES_MU RES_MU_S RES_NU texels x 2 NS^2 directions) and their issue rate
+        dirs = 2 * o.spherical_samples ** 2
+        roof["valu_insts_per_texel_direction"] = round(valu * 64 / (p.n_texels4d() * dirs), 2)
+        roof["valu_wave_insts_per_s"] = round(valu / (avg_ms * 1e-3), -6)
     if traffic:
         roof["traffic_source"] = TRAFFIC_PROFILE
         roof["hbm_gbs"] = round(traffic / (avg_ms * 1e-3) / 1e9, 1)
@@ -236,6 +284,10 @@ def main():
             "algorithmic_tflops": round(sum(F.total(q, orr) for q in sets) / secb / 1e12, 2)}
         log("batched 256 sets: %.3f s" % secb)
         del outsb
+    eng.close()
+    if not args.no_surrogate:
+        extra["surrogate"] = surrogate_block(args.sur_steps, 1)
+        extra["surrogate_exact_order_2"] = surrogate_block(args.sur_steps, 2)
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -260,7 +312,6 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         log("copied %d bytes" % (src.numel() * 4))
-    eng.close()
 
 
 if __name__ == "__main__":

@@ -3,16 +3,18 @@
  *
  * No counterpart in the reference (src/atmo/sources/proland/preprocess/atmo/
  * computes every scattering order with its GL passes).  A small MLP predicts
- * the contribution of scattering orders >= 2 to the inscatter and irradiance
- * tables; single scattering and transmittance stay exact (the plain passes).
- * The plain multi-order path is the judge and the fallback.
+ * the contribution of the higher scattering orders (>= 2 by default) to the
+ * inscatter and irradiance tables; transmittance and the lower orders stay
+ * exact (the plain passes).  The plain multi-order path is the judge and the
+ * fallback.
  *
  * Network: 11 inputs (4 texel coordinates in [-1, 1] + 7 normalised
  * atmosphere features) -> 4 x 256 SiLU -> 4 outputs (log-scaled rgb of the
- * orders >= 2 contribution, standardised; the 4th output is trained to 0
- * because orders >= 2 add nothing to alpha).  The three 256x256 layers run in
- * BF16 on the matrix cores with FP32 accumulation; master weights, Adam state,
- * the first and last layers are FP32.
+ * contribution of the orders above the exactly computed ones, standardised;
+ * the 4th output is trained to 0 because those orders add nothing to alpha).
+ * The three 256x256 layers run in BF16 on the matrix cores with FP32
+ * accumulation; master weights, Adam state, the first and last layers are
+ * FP32.
  *
  * Parameter vector layout (FP32): W1[11][256], b1[256], then for l = 2..4
  * Wl[256][256] (row = input), bl[256], then W5[256][4], b5[4].
@@ -37,6 +39,13 @@ ATMO_API size_t atmo_surrogate_param_count(void);
 ATMO_API atmo_status atmo_surrogate_get_params(atmo_surrogate *s, float *host, size_t n);
 ATMO_API atmo_status atmo_surrogate_set_params(atmo_surrogate *s, const float *host, size_t n);
 
+/* Scattering orders computed exactly by the plain passes (1: transmittance and
+ * single scattering, the default; 2: also order 2, ~half the plain path's
+ * cost); the network predicts the contribution of the orders above.  Set it
+ * before atmo_surrogate_build_dataset: the dataset's targets, training and
+ * atmo_surrogate_precompute all use it. */
+ATMO_API atmo_status atmo_surrogate_set_exact_orders(atmo_surrogate *s, int orders);
+
 /* Output standardisation: target_c = (log(max(v_c, 0) + eps) - mu_c) / sd_c. */
 ATMO_API atmo_status atmo_surrogate_get_norm(atmo_surrogate *s, float mu[3], float sd[3],
                                              float *eps);
@@ -44,7 +53,7 @@ ATMO_API atmo_status atmo_surrogate_set_norm(atmo_surrogate *s, const float mu[3
                                              const float sd[3], float eps);
 
 /* Training data from the plain method: each set is precomputed with `opts`
- * (orders >= 2) and with 1 order; `samples_per_set` seeded inscatter texels
+ * and with the exact orders only; `samples_per_set` seeded inscatter texels
  * and every irradiance texel of each set become rows.  Replaces any earlier
  * dataset; sets the output standardisation from the data. */
 ATMO_API atmo_status atmo_surrogate_build_dataset(atmo_surrogate *s, const atmo_params *sets,
@@ -79,14 +88,14 @@ typedef struct atmo_surrogate_report {
     int fallback_sets;    /* sets whose tables come from the plain method */
     float max_rel_err_S;  /* worst judged inscatter error (floored at 1e-4 of the table max) */
     float max_rel_err_E;  /* worst judged irradiance error */
-    float ms_exact;       /* exact transmittance + single scattering */
+    float ms_exact;       /* exact transmittance + the exactly computed orders */
     float ms_mlp;         /* network inference + combination */
     float ms_judge;       /* plain recompute of the judged sets + comparison */
     float ms_fallback;    /* plain recompute of the rejected batch */
 } atmo_surrogate_report;
 
 /* Surrogate-accelerated precompute of nsets sets into caller device tables:
- * exact transmittance and single scattering, network-predicted orders >= 2,
+ * exact transmittance and lower orders, network-predicted higher orders,
  * then a judge: ceil(judge_fraction * nsets) seeded sets are recomputed with
  * the plain multi-order path and compared.  If every judged set is within
  * `tolerance` (max relative error, floored at 1e-4 of the table maximum) the

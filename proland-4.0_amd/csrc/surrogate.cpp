@@ -97,6 +97,8 @@ FeatSet feat_set(const atmo_params &p) {
     f.f[4] = (p.HM - 1.5f) / 0.9f;
     f.f[5] = (float)((std::log((double)std::max(p.betaMSca.x, 1e-12f)) + 5.2072) / 1.7006);
     f.f[6] = (p.mieG - 0.75f) / 0.15f;
+    f.Rg = p.Rg;
+    f.Rt = p.Rt;
     return f;
 }
 
@@ -114,6 +116,7 @@ struct atmo_surrogate {
     Buf iset;                 // InferSet array of the last fused launch
     TargetNorm tn{{0, 0, 0}, {1, 1, 1}, 1e-6f};
     int adam_t = 0;
+    int exact_orders = 1;     // orders computed by the plain passes; the network predicts the rest
     Buf Xa, Ta;
     size_t nrows = 0;
     // work buffers: X, T, H, Y for `cap` rows; Z, dZ, dY for `zcap` rows
@@ -338,6 +341,12 @@ atmo_status atmo_surrogate_set_norm(atmo_surrogate *s, const float mu[3], const 
     return ATMO_OK;
 }
 
+atmo_status atmo_surrogate_set_exact_orders(atmo_surrogate *s, int orders) {
+    if (!s || orders < 1 || orders > 2) return fail(ATMO_ERR_INVALID_ARGUMENT, "exact orders must be 1 or 2");
+    s->exact_orders = orders;
+    return ATMO_OK;
+}
+
 atmo_status atmo_surrogate_features(const atmo_params *p, const uint32_t *codes, int n, float *X) {
     if (!p || !codes || !X || n <= 0) return fail(ATMO_ERR_INVALID_ARGUMENT, "bad arguments");
     HIPCHK(code_rows(feat_set(*p), codes, n, X, nullptr));
@@ -350,7 +359,9 @@ atmo_status atmo_surrogate_build_dataset(atmo_surrogate *s, const atmo_params *s
                                          size_t *rows_out) {
     if (!s || !sets || nsets <= 0 || !opts || samples_per_set <= 0)
         return fail(ATMO_ERR_INVALID_ARGUMENT, "bad arguments");
-    if (opts->orders < 2) return fail(ATMO_ERR_INVALID_ARGUMENT, "training data needs orders >= 2");
+    if (opts->orders <= s->exact_orders)
+        return fail(ATMO_ERR_INVALID_ARGUMENT, "training data needs more orders than the %d computed exactly",
+                    s->exact_orders);
     if (opts->flags & (ATMO_FLAG_HALF_STORAGE | ATMO_FLAG_LINEAR_TRANSMITTANCE | ATMO_FLAG_LINEAR_INSCATTER))
         return fail(ATMO_ERR_INVALID_ARGUMENT,
                     "the surrogate works on FP32 tables of the non-linear parameterisation "
@@ -362,10 +373,11 @@ atmo_status atmo_surrogate_build_dataset(atmo_surrogate *s, const atmo_params *s
     HIPCHK(s->Ta.ensure(rows * kNOUT * 4));
     atmo_options on = *opts, o1 = *opts;
     // every chunk of seeded sets is a new batch shape: no graph capture, no
-    // per-shape autotuning (the default kernel variants)
+    // per-shape autotuning (the default kernel variants, whose tables equal the
+    // tuned variants' bit for bit)
     on.flags |= ATMO_FLAG_NO_GRAPH | ATMO_FLAG_NO_AUTOTUNE;
-    o1.orders = 1;
-    o1.flags |= ATMO_FLAG_NO_GRAPH;
+    o1.orders = s->exact_orders;
+    o1.flags |= ATMO_FLAG_NO_GRAPH | ATMO_FLAG_NO_AUTOTUNE;
     const int CH = 16;
     TargetNorm raw{{0.f, 0.f, 0.f}, {1.f, 1.f, 1.f}, s->tn.eps};
     size_t row = 0;
@@ -388,7 +400,7 @@ atmo_status atmo_surrogate_build_dataset(atmo_surrogate *s, const atmo_params *s
         for (int i = 0; i < nc; ++i) {
             const atmo_params &p = sets[c0 + i];
             HIPCHK(make_rows(feat_set(p), (const float4 *)tn[i].inscatter, (const float4 *)t1[i].inscatter,
-                             tn[i].irradiance, samples_per_set, seed + (uint64_t)(c0 + i) * 7919u,
+                             tn[i].irradiance, t1[i].irradiance, samples_per_set, seed + (uint64_t)(c0 + i) * 7919u,
                              s->Xa.as<float>() + row * kXS, s->Ta.as<float>() + row * kNOUT, raw, s->st));
             row += (size_t)samples_per_set + (size_t)p.SKY_W * p.SKY_H;
         }
@@ -482,7 +494,7 @@ atmo_status atmo_surrogate_precompute(atmo_surrogate *s, const atmo_params *sets
                     "the surrogate works on FP32 tables of the non-linear parameterisation "
                     "(no ATMO_FLAG_HALF_STORAGE, ATMO_FLAG_LINEAR_*)");
     atmo_options o1 = o;
-    o1.orders = 1;
+    o1.orders = std::min(s->exact_orders, o.orders);
     hipStream_t hs = (hipStream_t)stream;
     atmo_surrogate_report r;
     memset(&r, 0, sizeof r);

@@ -24,6 +24,7 @@ struct GemmEpi {
 struct FeatSet {
     int RR, RMU, RMUS, RNU, SW, SH, n4, pad;
     float f[7];          // normalised atmosphere features
+    float Rg, Rt;        // planet and atmosphere radii (the mu-row coordinate's path lengths)
 };
 
 struct TargetNorm {
@@ -63,7 +64,8 @@ hipError_t reduce_partials(const float *P, int nsplit, int n, float *g, hipStrea
 hipError_t adam(float *p, const float *g, float *m, float *v, int n, float lr, float b1, float b2,
                 float eps, float c1, float c2, hipStream_t st);
 hipError_t to_bf16(const float *W, __bf16 *Wb, __bf16 *Wt, hipStream_t st);
-hipError_t make_rows(const FeatSet &fs, const float4 *S4, const float4 *S1, const float *E, int ns,
+// S1, E1: the tables of the exactly computed orders (the targets are the rest)
+hipError_t make_rows(const FeatSet &fs, const float4 *S4, const float4 *S1, const float *E, const float *E1, int ns,
                      unsigned long long seed, float *X, float *Tg, const TargetNorm &tn,
                      hipStream_t st);
 hipError_t infer_rows(const FeatSet &fs, int row0, int n, float *X, hipStream_t st);
@@ -78,7 +80,7 @@ hipError_t prep_fused(const float *W1, const float *b1, const float *W5, const f
                       hipStream_t st);
 // Fused inference of every S and E texel of a batch of sets (device array
 // `sets`, tile0 ascending, ntiles workgroups in all): adds the predicted
-// orders >= 2 to S.rgb and writes E.
+// higher orders to S.rgb and E.
 int fused_rows_per_block();
 hipError_t mlp_infer(const InferSet *sets, int nsets, int ntiles, const FusedWeights &fw, const TargetNorm &tn,
                      hipStream_t st);

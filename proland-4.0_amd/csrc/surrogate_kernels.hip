@@ -139,8 +139,9 @@ __global__ __launch_bounds__(256) void k_layer1_fwd(const float *__restrict__ X,
 }
 
 // ---------------------------------------------------------------------------
-// Layer 5 (256 -> 4), FP32 from bf16 H4, fused with the loss: one wave per
-// row, each lane 4 of the 256 inputs, shuffles for the dot products.
+// Layer 5 (256 -> 4), FP32 from bf16 H4, fused with the loss: a wave takes a
+// row at a time (16 rows), each lane 4 of the 256 inputs, shuffles for the
+// dot products.
 //   mode 0 (train):   dY = 2 (Y - T) / (rows * 4), loss += sum (Y - T)^2 / (rows * 4)
 //   mode 1 (predict): Y written
 // ---------------------------------------------------------------------------
@@ -150,41 +151,49 @@ __global__ __launch_bounds__(256) void k_layer5(const __bf16 *__restrict__ H4,
                                                 const float *__restrict__ Tg, int rows, int mode,
                                                 float *__restrict__ Y, float *__restrict__ dY,
                                                 float *__restrict__ loss) {
-    const int lane = threadIdx.x & 63;
-    const size_t b = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (b >= (size_t)rows) return;
-    float acc[kNOUT] = {0.f, 0.f, 0.f, 0.f};
+    // a block takes 64 rows, 16 per wave; the loss of its rows is summed in the
+    // block and added with one atomic (one per row serialised on the loss word)
+    __shared__ float lsum[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const float inv = 1.0f / ((float)rows * kNOUT);
+    float l = 0.f;
+    for (int rr = 0; rr < 16; ++rr) {
+        const size_t b = (size_t)blockIdx.x * 64 + wv * 16 + rr;
+        if (b >= (size_t)rows) break;
+        float acc[kNOUT] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int k = lane * 4 + q;
-        const float h = (float)H4[b * kNH + k];
+        for (int q = 0; q < 4; ++q) {
+            const int k = lane * 4 + q;
+            const float h = (float)H4[b * kNH + k];
 #pragma unroll
-        for (int o = 0; o < kNOUT; ++o) acc[o] = fmaf(h, W5[k * kNOUT + o], acc[o]);
-    }
+            for (int o = 0; o < kNOUT; ++o) acc[o] = fmaf(h, W5[k * kNOUT + o], acc[o]);
+        }
 #pragma unroll
-    for (int o = 0; o < kNOUT; ++o)
+        for (int o = 0; o < kNOUT; ++o)
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) acc[o] += __shfl_xor(acc[o], off, 64);
-    if (lane == 0) {
-        const float inv = 1.0f / ((float)rows * kNOUT);
-        float l = 0.f;
+            for (int off = 32; off > 0; off >>= 1) acc[o] += __shfl_xor(acc[o], off, 64);
+        if (lane == 0) {
 #pragma unroll
-        for (int o = 0; o < kNOUT; ++o) {
-            const float y = acc[o] + b5[o];
-            if (mode == 0) {
-                const float d = y - Tg[b * kNOUT + o];
-                dY[b * kNOUT + o] = 2.0f * d * inv;
-                l += d * d;
-            } else {
-                Y[b * kNOUT + o] = y;
+            for (int o = 0; o < kNOUT; ++o) {
+                const float y = acc[o] + b5[o];
+                if (mode == 0) {
+                    const float d = y - Tg[b * kNOUT + o];
+                    dY[b * kNOUT + o] = 2.0f * d * inv;
+                    l += d * d;
+                } else {
+                    Y[b * kNOUT + o] = y;
+                }
             }
         }
-        if (mode == 0) atomicAdd(loss, l * inv);
     }
+    if (mode != 0) return;
+    if (lane == 0) lsum[wv] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss, (lsum[0] + lsum[1] + lsum[2] + lsum[3]) * inv);
 }
 
 // Layer 5 backward: dW5 += H4^T dY, db5 += sum dY, dZ4 = (dY W5^T) * SiLU'(Z4).
-// One wave per row as in the forward.
+// A wave per row at a time as in the forward.
 __global__ __launch_bounds__(256) void k_layer5_bwd(const __bf16 *__restrict__ H4,
                                                     const float *__restrict__ Z4,
                                                     const float *__restrict__ W5,
@@ -297,6 +306,8 @@ __global__ __launch_bounds__(256) void k_to_bf16(const float *__restrict__ W, __
 // ---------------------------------------------------------------------------
 // Features of table texels and dataset assembly
 // ---------------------------------------------------------------------------
+constexpr float kHorizonGap = 0.25f;
+
 // Resolution-independent texel coordinates in [-1, 1] (the arguments of
 // getMuMuSNu, common.glsl:175-201) plus the 7 normalised atmosphere features.
 // E texels use the irradiance coordinates in slots 0 and 2 and -2 in 1 and 3.
@@ -311,10 +322,37 @@ __device__ __forceinline__ void texel_features(const FeatSet &fs, unsigned code,
     } else {
         const int W4 = fs.RMUS * fs.RNU;
         const int xx = code % W4, y = (code / W4) % fs.RMU, l = code / (W4 * fs.RMU);
+        // mu rows: the row's ray length d (getMuMuSNu, common.glsl:175-190, with
+        // the layer radius of setLayer, PreprocessAtmo.cpp:255-271) on one log
+        // scale for the whole set, from 1 m (the shortest ray of the table: up
+        // from the highest layer) to 2 sqrt(Rt^2 - Rg^2) (longer than any ray),
+        // the ground-ray half in [-1, -kHorizonGap] and the sky half in
+        // [kHorizonGap, 1].  The table rows are linear in d up to a clamp at the
+        // shortest ray of a layer (10 m down from the lowest layer), where the
+        // inscatter, proportional to d, drops by 10-100x from one row to the
+        // next; on a log scale common to all layers that drop is the same slope
+        // everywhere, and the gap separates the straight-down and straight-up
+        // rows, which are neighbours in the table only.
         const int half = fs.RMU / 2;
-        const float umu = y < half ? 0.5f * y / (half - 1) : 0.5f + 0.5f * (y - half) / (half - 1);
+        const double Rg = fs.Rg, Rt = fs.Rt, Rg2 = Rg * Rg;
+        double rl = (double)l / (fs.RR - 1);
+        rl = sqrt(Rg2 + rl * rl * (Rt * Rt - Rg2)) + (l == 0 ? 0.01 : (l == fs.RR - 1 ? -0.001 : 0.0));
+        const double rho = sqrt(fmax(rl * rl - Rg2, 0.0)), H = sqrt(Rt * Rt - Rg2);
+        double dmin, dmax, d;
+        if (y < half) {
+            dmin = rl - Rg;
+            dmax = 0.999 * rho;
+            d = (1.0 - (double)y / (half - 1)) * rho;
+        } else {
+            dmin = Rt - rl;
+            dmax = 0.999 * (rho + H);
+            d = (double)(y - half) / (half - 1) * (rho + H);
+        }
+        d = fmin(fmax(d, dmin), dmax);
+        const double dlo = 1e-3, dhi = 2.0 * H;   // in the parameters' length unit (km)
+        const float t = (float)fmin(fmax(log(d / dlo) / log(dhi / dlo), 0.0), 1.0);
         x[0] = 2.f * l / (fs.RR - 1) - 1.f;
-        x[1] = 2.f * umu - 1.f;
+        x[1] = y < half ? fmaf(-(1.f - kHorizonGap), t, -kHorizonGap) : fmaf(1.f - kHorizonGap, t, kHorizonGap);
         x[2] = 2.f * (xx % fs.RMUS) / (fs.RMUS - 1) - 1.f;
         x[3] = 2.f * (xx / fs.RMUS) / (fs.RNU - 1) - 1.f;
     }
@@ -330,11 +368,13 @@ __device__ __forceinline__ unsigned hash32(unsigned long long z) {
 }
 
 // Training rows of one set: `ns` seeded S texels then all E texels.
-// target c = (log(max(v_c, 0) + eps) - mu_c) / sd_c, v = S(orders<=N) - S(order 1) for S
-// texels (rgb) and E for E texels; the fourth target is 0 (orders >= 2 add 0 to alpha).
+// target c = (log(max(v_c, 0) + eps) - mu_c) / sd_c, v = S(orders<=N) - S(orders<=K) for S
+// texels (rgb) and E(orders<=N) - E(orders<=K) for E texels, K the exactly computed
+// orders (S1, E1); the fourth target is 0 (the orders above 1 add 0 to alpha).
 __global__ __launch_bounds__(256) void k_make_rows(FeatSet fs, const float4 *__restrict__ S4,
                                                    const float4 *__restrict__ S1,
-                                                   const float *__restrict__ E, int ns,
+                                                   const float *__restrict__ E,
+                                                   const float *__restrict__ E1, int ns,
                                                    unsigned long long seed, float *__restrict__ X,
                                                    float *__restrict__ Tg, TargetNorm tn) {
     const int t = blockIdx.x * 256 + threadIdx.x;
@@ -349,7 +389,7 @@ __global__ __launch_bounds__(256) void k_make_rows(FeatSet fs, const float4 *__r
     } else {
         const int e = t - ns;
         code = 0x80000000u | (unsigned)e;
-        v[0] = E[3 * e]; v[1] = E[3 * e + 1]; v[2] = E[3 * e + 2];
+        v[0] = E[3 * e] - E1[3 * e]; v[1] = E[3 * e + 1] - E1[3 * e + 1]; v[2] = E[3 * e + 2] - E1[3 * e + 2];
     }
     float x[kNIN];
     texel_features(fs, code, x);
@@ -425,8 +465,8 @@ __global__ __launch_bounds__(256) void k_norm_targets(float *__restrict__ Tg, si
     for (int c = 0; c < 3; ++c) Tg[t * kNOUT + c] = (Tg[t * kNOUT + c] - tn.mu[c]) / tn.sd[c];
 }
 
-// Adds the predicted orders >= 2 to the exact single scattering (S.rgb) and
-// sets E.  Y rows follow k_infer_rows.
+// Adds the predicted higher orders to the exact part (S.rgb and E).  Y rows
+// follow k_infer_rows.
 __global__ __launch_bounds__(256) void k_apply(int row0, int n, int n4, const float *__restrict__ Y,
                                                TargetNorm tn, float4 *__restrict__ S,
                                                float *__restrict__ E) {
@@ -442,7 +482,7 @@ __global__ __launch_bounds__(256) void k_apply(int row0, int n, int n4, const fl
         S[r] = s;
     } else {
         const int e = r - n4;
-        E[3 * e] = v[0]; E[3 * e + 1] = v[1]; E[3 * e + 2] = v[2];
+        E[3 * e] += v[0]; E[3 * e + 1] += v[1]; E[3 * e + 2] += v[2];
     }
 }
 
@@ -736,7 +776,7 @@ __global__ __launch_bounds__(64 * NW) void k_mlp_infer(const InferSet *__restric
                 S[r] = sv;
             } else {
                 const int e = r - fs.n4;
-                E[3 * e] = v[0]; E[3 * e + 1] = v[1]; E[3 * e + 2] = v[2];
+                E[3 * e] += v[0]; E[3 * e + 1] += v[1]; E[3 * e + 2] += v[2];
             }
         }
     }
@@ -795,7 +835,7 @@ hipError_t layer1_fwd(const float *X, const float *W1, const float *b1, int rows
 }
 hipError_t layer5(const __bf16 *H4, const float *W5, const float *b5, const float *Tg, int rows, int mode,
                   float *Y, float *dY, float *loss, hipStream_t st) {
-    hipLaunchKernelGGL(k_layer5, dim3((rows + 3) / 4), dim3(256), 0, st, H4, W5, b5, Tg, rows, mode, Y, dY, loss);
+    hipLaunchKernelGGL(k_layer5, dim3((rows + 63) / 64), dim3(256), 0, st, H4, W5, b5, Tg, rows, mode, Y, dY, loss);
     return hipGetLastError();
 }
 hipError_t layer5_bwd(const __bf16 *H4, const float *Z4, const float *W5, const float *dY, int rows,
@@ -804,12 +844,12 @@ hipError_t layer5_bwd(const __bf16 *H4, const float *Z4, const float *W5, const 
     return hipGetLastError();
 }
 hipError_t layer1_bwd(const float *X, const float *dZ1, int rows, float *gW1, float *gb1, hipStream_t st) {
-    const int chunk = 256;
+    const int chunk = 64;   // 1024 workgroups at the training batch
     hipLaunchKernelGGL(k_layer1_bwd, dim3((rows + chunk - 1) / chunk), dim3(256), 0, st, X, dZ1, rows, chunk, gW1, gb1);
     return hipGetLastError();
 }
 hipError_t colsum(const __bf16 *D, int rows, float *g, hipStream_t st) {
-    const int chunk = 256;
+    const int chunk = 64;   // 1024 workgroups at the training batch
     hipLaunchKernelGGL(k_colsum, dim3((rows + chunk - 1) / chunk), dim3(256), 0, st, D, rows, chunk, g);
     return hipGetLastError();
 }
@@ -826,10 +866,10 @@ hipError_t to_bf16(const float *W, __bf16 *Wb, __bf16 *Wt, hipStream_t st) {
     hipLaunchKernelGGL(k_to_bf16, dim3(blocks(kNH * kNH)), dim3(256), 0, st, W, Wb, Wt);
     return hipGetLastError();
 }
-hipError_t make_rows(const FeatSet &fs, const float4 *S4, const float4 *S1, const float *E, int ns,
+hipError_t make_rows(const FeatSet &fs, const float4 *S4, const float4 *S1, const float *E, const float *E1, int ns,
                      unsigned long long seed, float *X, float *Tg, const TargetNorm &tn, hipStream_t st) {
     const int n = ns + fs.SW * fs.SH;
-    hipLaunchKernelGGL(k_make_rows, dim3(blocks(n)), dim3(256), 0, st, fs, S4, S1, E, ns, seed, X, Tg, tn);
+    hipLaunchKernelGGL(k_make_rows, dim3(blocks(n)), dim3(256), 0, st, fs, S4, S1, E, E1, ns, seed, X, Tg, tn);
     return hipGetLastError();
 }
 hipError_t infer_rows(const FeatSet &fs, int row0, int n, float *X, hipStream_t st) {

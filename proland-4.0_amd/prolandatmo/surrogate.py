@@ -29,6 +29,7 @@ SIGNATURES = {
     "atmo_surrogate_param_count": (ctypes.c_size_t, []),
     "atmo_surrogate_get_params": (ctypes.c_int, [_vp, _fp, ctypes.c_size_t]),
     "atmo_surrogate_set_params": (ctypes.c_int, [_vp, _fp, ctypes.c_size_t]),
+    "atmo_surrogate_set_exact_orders": (ctypes.c_int, [_vp, ctypes.c_int]),
     "atmo_surrogate_get_norm": (ctypes.c_int, [_vp, _fp, _fp, _fp]),
     "atmo_surrogate_set_norm": (ctypes.c_int, [_vp, _fp, _fp, ctypes.c_float]),
     "atmo_surrogate_build_dataset": (ctypes.c_int, [_vp, ctypes.POINTER(AtmoParams), ctypes.c_int,
@@ -104,6 +105,10 @@ class Surrogate:
     def set_params(self, a):
         a = np.ascontiguousarray(a, dtype=np.float32)
         L.check(lib().atmo_surrogate_set_params(self.h, a.ctypes.data_as(_fp), a.size))
+
+    def set_exact_orders(self, n):
+        """Orders computed by the plain passes (1 or 2); the network predicts the rest."""
+        L.check(lib().atmo_surrogate_set_exact_orders(self.h, int(n)))
 
     def get_norm(self):
         mu, sd, eps = (ctypes.c_float * 3)(), (ctypes.c_float * 3)(), ctypes.c_float()
