@@ -204,6 +204,28 @@ ATMO_API atmo_status atmo_engine_tuning(atmo_engine *e, atmo_tuning *out);
 ATMO_API atmo_status atmo_engine_workspace_size(const atmo_params *sets, int nsets,
                                                 const atmo_options *opts, size_t *bytes);
 
+/* Allocation-free precompute.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, tables and semantics as
+ * atmo_engine_precompute, but every device buffer it needs (work tables,
+ * set descriptors, tile and direction tables, check counters) is a part of
+ * one static device array of ATMO_NOALLOC_MAX_BYTES per GPU, which the
+ * library holds from its first call on a GPU until the process ends; it
+ * calls no hipMalloc / hipFree.  A batch that needs more returns
+ * ATMO_ERR_INVALID_ARGUMENT before any device work (one set at the reference
+ * configuration, 32x128x32x8 with T 256x64 and E 64x16, needs 48.4 MiB).
+ * The caller passes the output tables.  Runs on the current device and the
+ * caller's stream, launches eagerly (no hipGraph) and never autotunes: it
+ * takes the kernel variants an engine in this process tuned for the same
+ * shape class (then its tables equal that engine's bit for bit), else the
+ * defaults (ATMO_FLAG_NO_AUTOTUNE's tables).  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (other threads, other
+ * streams) race; calls on one stream are ordered. */
+#define ATMO_NOALLOC_MAX_BYTES (64u * 1024u * 1024u)
+ATMO_API atmo_status atmo_precompute_noalloc(const atmo_params *sets, int nsets,
+                                             const atmo_options *opts, const atmo_tables *out,
+                                             void *stream);
+
 /* ---------------------------------------------------------------------- */
 /* Pass-level entry points (one reference shader program each)             */
 /* All pointers: device float4 tables on the engine's device.              */

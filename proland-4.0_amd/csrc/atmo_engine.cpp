@@ -244,17 +244,38 @@ bool same_opts(const atmo_options &a, const atmo_options &b) {
 struct DevBuf {
     void *p = nullptr;
     size_t n = 0;
+    bool owned = true;   // false: a part of memory the buffer neither frees nor grows
     ~DevBuf() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
     }
     hipError_t ensure(size_t bytes) {
         if (bytes <= n) return hipSuccess;
+        if (!owned) return hipErrorOutOfMemory;
         if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
         hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 256));
         if (e == hipSuccess) n = bytes;
         return e;
+    }
+};
+
+// Where a plan's device buffers come from: hipMalloc (base == nullptr), or
+// consecutive 256-byte aligned parts of one caller-provided device region
+// (the allocation-free entry point).
+struct Region {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    hipError_t get(DevBuf &b, size_t bytes) {
+        if (!base) return b.ensure(bytes);
+        bytes = align256(std::max<size_t>(bytes, 1));
+        if (used + bytes > cap) return hipErrorOutOfMemory;
+        if (b.p && b.owned) (void)hipFree(b.p);
+        b.p = base + used;
+        b.n = bytes;
+        b.owned = false;
+        used += bytes;
+        return hipSuccess;
     }
 };
 
@@ -413,13 +434,14 @@ size_t dirs_bytes(const atmo_options &o, size_t &offJT, size_t &offJW, size_t &o
 }
 
 atmo_status upload_dirs(DevBuf &buf, hipStream_t xs, const atmo_options &o, size_t &offJT, size_t &offJW,
-                        size_t &offE) {
+                        size_t &offE, Region *rg = nullptr) {
     std::vector<DirJHost> jt;
     std::vector<float> jw;
     std::vector<DirEHost> de;
     build_dirs(o.spherical_samples, o.irradiance_samples, jt, jw, de);
     const size_t total = dirs_bytes(o, offJT, offJW, offE);
-    HIPCHK(buf.ensure(total));
+    Region none;
+    HIPCHK((rg ? rg : &none)->get(buf, total));
     std::vector<char> h(total, 0);
     memcpy(h.data() + offJT, jt.data(), jt.size() * sizeof(DirJHost));
     memcpy(h.data() + offJW, jw.data(), jw.size() * sizeof(float));
@@ -762,14 +784,25 @@ atmo_status apply_tuning(Plan &pl, const Tuning &tu) {
     return ATMO_OK;
 }
 
+// rg: where the device buffers come from (nullptr: hipMalloc); tu: kernel
+// variants to build the plan with (nullptr: the defaults, tuned later).
 atmo_status build_plan(hipStream_t xs, const atmo_params *sets, int nsets, const atmo_options &o,
-                       std::unique_ptr<Plan> &out) {
+                       std::unique_ptr<Plan> &out, Region *rg = nullptr, const Tuning *tu = nullptr) {
     std::unique_ptr<Plan> pl(new Plan);
+    Region heap;
+    Region &R = rg ? *rg : heap;
     pl->xs = xs;
     pl->params.assign(sets, sets + nsets);
     pl->opts = o;
     pl->jG = o.texels_per_thread;   // 0: each set with its own G until the tuner has run
     pl->snSb = form_sb(o.inscatterN_form);
+    if (tu) {
+        pl->jG = o.texels_per_thread ? o.texels_per_thread : tu->jG;
+        pl->snCap = tu->snCap;
+        pl->snXcd = tu->snXcd;
+        pl->snSb = o.inscatterN_form ? form_sb(o.inscatterN_form) : tu->snSb;
+        pl->tuned = true;
+    }
     // work arena
     size_t total = 0;
     std::vector<size_t> base(nsets);
@@ -777,7 +810,7 @@ atmo_status build_plan(hipStream_t xs, const atmo_params *sets, int nsets, const
         base[i] = total;
         total += work_bytes(sets[i], o);
     }
-    HIPCHK(pl->arena.ensure(total));
+    HIPCHK(R.get(pl->arena, total));
     // layers
     std::vector<LayerGeom> layers;
     std::vector<size_t> loff(nsets);
@@ -785,7 +818,7 @@ atmo_status build_plan(hipStream_t xs, const atmo_params *sets, int nsets, const
         loff[i] = layers.size();
         for (int l = 0; l < sets[i].RES_R; ++l) layers.push_back(layer_geom(sets[i], l));
     }
-    HIPCHK(pl->dlayers.ensure(layers.size() * sizeof(LayerGeom)));
+    HIPCHK(R.get(pl->dlayers, layers.size() * sizeof(LayerGeom)));
     HIPCHK(h2d(pl->dlayers.p, layers.data(), layers.size() * sizeof(LayerGeom), pl->xs));
     // descriptors (output pointers are filled in per call)
     pl->hsets.resize(nsets);
@@ -811,14 +844,14 @@ atmo_status build_plan(hipStream_t xs, const atmo_params *sets, int nsets, const
         set_table_flags(s, o);
         s.layers = (const LayerGeom *)pl->dlayers.p + loff[i];
     }
-    HIPCHK(pl->dsets.ensure(nsets * sizeof(SetDesc)));
+    HIPCHK(R.get(pl->dsets, nsets * sizeof(SetDesc)));
     // tiles
     std::vector<uint2> tiles;
     build_tiles(pl->params, o, tiles, pl->off, pl->cnt, pl->snCap, pl->snXcd, &pl->jseg);
-    HIPCHK(pl->dtiles.ensure(tiles.size() * sizeof(uint2)));
+    HIPCHK(R.get(pl->dtiles, tiles.size() * sizeof(uint2)));
     HIPCHK(h2d(pl->dtiles.p, tiles.data(), tiles.size() * sizeof(uint2), pl->xs));
-    HIPCHK(pl->dflags.ensure(nsets * sizeof(unsigned)));
-    atmo_status st = upload_dirs(pl->ddirs, pl->xs, o, pl->dirJT_off, pl->dirJW_off, pl->dirE_off);
+    HIPCHK(R.get(pl->dflags, nsets * sizeof(unsigned)));
+    atmo_status st = upload_dirs(pl->ddirs, pl->xs, o, pl->dirJT_off, pl->dirJW_off, pl->dirE_off, &R);
     if (st != ATMO_OK) return st;
     int maxNU = 0, maxSW = 0;
     for (int i = 0; i < nsets; ++i) {
@@ -839,6 +872,23 @@ atmo_status build_plan(hipStream_t xs, const atmo_params *sets, int nsets, const
     pl->ldsEN = lds_irradianceN(pl->chEN, maxNU, false);
     out = std::move(pl);
     return ATMO_OK;
+}
+
+// Device bytes build_plan takes from a Region for this batch (every part
+// rounded to 256 bytes, as Region::get does).
+size_t plan_bytes(const std::vector<atmo_params> &ps, const atmo_options &o, int snCap, int snXcd) {
+    size_t work = 0, nl = 0;
+    for (const atmo_params &p : ps) {
+        work += work_bytes(p, o);
+        nl += (size_t)p.RES_R;
+    }
+    std::vector<uint2> tiles;
+    int off[P_COUNT], cnt[P_COUNT];
+    build_tiles(ps, o, tiles, off, cnt, snCap, snXcd);
+    size_t a, b, c;
+    return align256(work) + align256(nl * sizeof(LayerGeom)) + align256(ps.size() * sizeof(SetDesc)) +
+           align256(tiles.size() * sizeof(uint2)) + align256(ps.size() * sizeof(unsigned)) +
+           align256(dirs_bytes(o, a, b, c));
 }
 
 // Runs every pass of a plan on the caller's stream: with pass timing (eager
@@ -917,6 +967,39 @@ atmo_status check_numerics(Plan &pl, hipStream_t hs) {
     return ATMO_OK;
 }
 
+}  // namespace
+
+// ===========================================================================
+// Work memory of the allocation-free precompute (atmo_precompute_noalloc):
+// one file-scope device array, which every GPU the library runs on holds (its
+// own copy, at its own address) until the process ends.  Host code never
+// passes `scratch` itself to a kernel or a copy (in host code the name is a
+// host-side stand-in, not a device address): it uses the address
+// hipGetSymbolAddress returns for the current device.
+// ===========================================================================
+static __device__ __align__(256) unsigned char scratch[ATMO_NOALLOC_MAX_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up on the first
+// call on that device.
+atmo_status scratch_ptr(char **out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(ATMO_ERR_DEVICE, "device %d: no work array slot", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(ATMO_ERR_DEVICE, "device %d: hipGetSymbolAddress gave a null work array", dev);
+        g_scratch[dev] = (char *)p;
+    }
+    *out = g_scratch[dev];
+    return ATMO_OK;
+}
 }  // namespace
 
 // ===========================================================================
@@ -1089,6 +1172,63 @@ atmo_status atmo_engine_precompute(atmo_engine *e, const atmo_params *sets, int 
     }
     if (o.flags & ATMO_FLAG_CHECK_NUMERICS)
         if ((st = check_numerics(pl, hs)) != ATMO_OK) return st;
+    g_err.clear();
+    return ATMO_OK;
+}
+
+atmo_status atmo_precompute_noalloc(const atmo_params *sets, int nsets, const atmo_options *opts,
+                                    const atmo_tables *out, void *stream) {
+    if (!sets || nsets <= 0 || !out) return fail(ATMO_ERR_INVALID_ARGUMENT, "bad arguments");
+    atmo_options o = opts ? *opts : default_options();
+    atmo_status st = validate_opts(o);
+    if (st != ATMO_OK) return st;
+    for (int i = 0; i < nsets; ++i) {
+        if ((st = validate(sets[i], i)) != ATMO_OK) return st;
+        if ((st = check_out(out[i], i)) != ATMO_OK) return st;
+    }
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if ((st = check_device(dev)) != ATMO_OK) return st;
+    // the variants of this shape class's cached tuning (an engine in this
+    // process tuned it), else the defaults; no autotuning here (it allocates)
+    std::vector<atmo_params> ps(sets, sets + nsets);
+    Tuning tu;
+    bool have = false;
+    if (o.orders >= 2 && !(o.flags & ATMO_FLAG_NO_AUTOTUNE)) {
+        std::lock_guard<std::mutex> lk(tuning_mutex());
+        auto it = tuning_cache().find(shape_key(ps, o));
+        if (it != tuning_cache().end()) {
+            tu = it->second;
+            have = true;
+        }
+    }
+    const int snCap = have ? tu.snCap : 4, snXcd = have ? tu.snXcd : 1;
+    const size_t need = plan_bytes(ps, o, snCap, snXcd);
+    if (need > (size_t)ATMO_NOALLOC_MAX_BYTES)
+        return fail(ATMO_ERR_INVALID_ARGUMENT,
+                    "this batch needs %zu bytes of device work memory; the allocation-free precompute "
+                    "holds at most %zu (ATMO_NOALLOC_MAX_BYTES): use atmo_engine_precompute",
+                    need, (size_t)ATMO_NOALLOC_MAX_BYTES);
+    char *base = nullptr;
+    if ((st = scratch_ptr(&base)) != ATMO_OK) return st;
+    hipStream_t hs = (hipStream_t)stream;
+    // every upload below is ordered after the earlier work of the caller's
+    // stream, which may still read the array
+    Region rg;
+    rg.base = base;
+    rg.cap = ATMO_NOALLOC_MAX_BYTES;
+    std::unique_ptr<Plan> pl;
+    if ((st = build_plan(hs, sets, nsets, o, pl, &rg, have ? &tu : nullptr)) != ATMO_OK) return st;
+    for (int i = 0; i < nsets; ++i) {
+        SetDesc &d = pl->hsets[i];
+        d.S4 = (float4 *)out[i].inscatter;
+        d.outT = out[i].transmittance;
+        d.outE = out[i].irradiance;
+    }
+    HIPCHK(h2d(pl->dsets.p, pl->hsets.data(), nsets * sizeof(SetDesc), hs));
+    if ((st = launch_all(*pl, hs)) != ATMO_OK) return st;
+    if (o.flags & ATMO_FLAG_CHECK_NUMERICS)
+        if ((st = check_numerics(*pl, hs)) != ATMO_OK) return st;
     g_err.clear();
     return ATMO_OK;
 }

@@ -75,6 +75,8 @@ SIGNATURES = {
     "atmo_engine_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(AtmoTuning)]),
     "atmo_engine_workspace_size": (ctypes.c_int, [_pp, ctypes.c_int, _po,
                                                   ctypes.POINTER(ctypes.c_size_t)]),
+    "atmo_precompute_noalloc": (ctypes.c_int, [_pp, ctypes.c_int, _po, ctypes.POINTER(AtmoTables),
+                                               _vp]),
     "atmo_pass_transmittance": (ctypes.c_int, [_vp, _pp, _po, _vp, _vp]),
     "atmo_pass_irradiance1": (ctypes.c_int, [_vp, _pp, _po, _vp, _vp, _vp]),
     "atmo_pass_inscatter1": (ctypes.c_int, [_vp, _pp, _po, _vp, _vp, _vp, _vp, _vp]),
@@ -217,6 +219,27 @@ class Engine:
     def pass_inscatterN(self, p, opts, T4, dJ4, dSR4, S4=None, stream=None):
         check(lib().atmo_pass_inscatterN(self.h, p, opts, _ptr(T4), _ptr(dJ4), _ptr(dSR4),
                                          _ptr(S4), _stream(stream)))
+
+
+# bytes of the allocation-free precompute's static device array (ATMO_NOALLOC_MAX_BYTES)
+NOALLOC_MAX_BYTES = 64 * 1024 * 1024
+
+
+def precompute_noalloc(sets, opts=None, outs=None, stream=None):
+    """atmo_precompute_noalloc: the batched precompute with its work tables in
+    the library's static device array (no device allocation by the library).
+    outs: list of (T, E, S) tensors (allocated here if None).  Returns them."""
+    if isinstance(sets, AtmoParams):
+        sets = [sets]
+    opts = opts or default_options()
+    if outs is None:
+        outs = [alloc_outputs(p) for p in sets]
+    tabs = (AtmoTables * len(sets))()
+    for i, (T, E, S) in enumerate(outs):
+        tabs[i] = AtmoTables(T.data_ptr(), E.data_ptr(), S.data_ptr())
+    check(lib().atmo_precompute_noalloc(params_array(sets), len(sets), ctypes.byref(opts), tabs,
+                                        _stream(stream)))
+    return outs
 
 
 def workspace_size(sets, opts=None):
