@@ -83,8 +83,9 @@ typedef struct atmo_options {
                                    entry points, autotuned in atmo_engine_precompute */
     int inscatterN_form;        /* k_inscatterN form: 1 direct gathers, 2 LDS-staged rows
                                    (2 samples per LDS round), 3 LDS-staged rows (1 sample
-                                   per round); 0 = direct for the pass-level entry points,
-                                   autotuned in atmo_engine_precompute */
+                                   per round), all bit-identical; 0 = form 3 for the
+                                   pass-level entry points, autotuned in
+                                   atmo_engine_precompute */
     int reserved[8];
 } atmo_options;
 

@@ -8,15 +8,16 @@
  * exact (the plain passes).  The plain multi-order path is the judge and the
  * fallback.
  *
- * Network: 11 inputs (4 texel coordinates in [-1, 1] + 7 normalised
- * atmosphere features) -> 4 x 256 SiLU -> 4 outputs (log-scaled rgb of the
+ * Network: 14 inputs (4 texel coordinates in [-1, 1], 7 normalised
+ * atmosphere features, and the log-scaled rgb of the exactly computed orders
+ * at the texel) -> 4 x 256 SiLU -> 4 outputs (log-scaled rgb of the
  * contribution of the orders above the exactly computed ones, standardised;
  * the 4th output is trained to 0 because those orders add nothing to alpha).
  * The three 256x256 layers run in BF16 on the matrix cores with FP32
  * accumulation; master weights, Adam state, the first and last layers are
  * FP32.
  *
- * Parameter vector layout (FP32): W1[11][256], b1[256], then for l = 2..4
+ * Parameter vector layout (FP32): W1[14][256], b1[256], then for l = 2..4
  * Wl[256][256] (row = input), bl[256], then W5[256][4], b5[4].
  */
 #ifndef ATMO_SURROGATE_H_
@@ -34,7 +35,7 @@ typedef struct atmo_surrogate atmo_surrogate;
 ATMO_API atmo_status atmo_surrogate_create(atmo_surrogate **out, int device, uint64_t seed);
 ATMO_API atmo_status atmo_surrogate_destroy(atmo_surrogate *s);
 
-/* Number of FP32 parameters (201476) and host copies of them. */
+/* Number of FP32 parameters (202244) and host copies of them. */
 ATMO_API size_t atmo_surrogate_param_count(void);
 ATMO_API atmo_status atmo_surrogate_get_params(atmo_surrogate *s, float *host, size_t n);
 ATMO_API atmo_status atmo_surrogate_set_params(atmo_surrogate *s, const float *host, size_t n);
@@ -77,8 +78,11 @@ ATMO_API atmo_status atmo_surrogate_loss_grad(atmo_surrogate *s, const float *X,
 ATMO_API atmo_status atmo_surrogate_predict(atmo_surrogate *s, const float *X, int B, float *Y);
 
 /* Features of texels of one set (codes < n4: inscatter texel index; codes with
- * bit 31 set: irradiance texel index) into device X[n][16]. */
-ATMO_API atmo_status atmo_surrogate_features(const atmo_params *p, const uint32_t *codes, int n,
+ * bit 31 set: irradiance texel index) into device X[n][16].  S_exact, E_exact:
+ * the set's device inscatter (float4) and irradiance (float3) tables of the
+ * exactly computed orders (the plain path with that many orders). */
+ATMO_API atmo_status atmo_surrogate_features(const atmo_params *p, const float *S_exact,
+                                             const float *E_exact, const uint32_t *codes, int n,
                                              float *X);
 
 typedef struct atmo_surrogate_report {

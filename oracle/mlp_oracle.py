@@ -9,7 +9,7 @@ kernels store them, so the two agree up to accumulation order.
 """
 import numpy as np
 
-NIN, XS, NH, NOUT = 11, 16, 256, 4
+NIN, XS, NH, NOUT = 14, 16, 256, 4
 
 
 def layout():
@@ -55,7 +55,7 @@ def dsilu(z):
 
 
 def forward(p, X, emulate_bf16=True):
-    """Returns (Y, cache).  X: (B, >=11)."""
+    """Returns (Y, cache).  X: (B, >=14)."""
     r = bf16 if emulate_bf16 else (lambda a: np.asarray(a, dtype=np.float64))
     w = unpack(p)
     X = np.asarray(X, dtype=np.float64)[:, :NIN]

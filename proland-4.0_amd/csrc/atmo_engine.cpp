@@ -304,13 +304,14 @@ struct JSeg {   // a run of P_J tiles whose sets share one G
 
 // Kernel variants chosen by the autotuner, with the times measured for each.
 struct Tuning {
-    int jG = 4, snCap = 4, snXcd = 1, snSb = 0;   // jG = 0: per-set choice
+    int jG = 4, snCap = 4, snXcd = 1, snSb = 1;   // jG = 0: per-set choice
     float msJ[5] = {0, 0, 0, 0, 0};  // G = 2, 3, 4, 8, per set (order-2 + order->=3 launches)
     float msN[27] = {};   // [direct, staged 1, staged 2][caps 1, 2, 4][plain, XCD, XCD + line order]
 };
 
-// staged samples per LDS round of a pinned k_inscatterN form (1 direct: 0)
-int form_sb(int form) { return form == 2 ? 2 : (form == 3 ? 1 : 0); }
+// staged samples per LDS round of a k_inscatterN form (1 direct: 0; 0, the
+// default before tuning: the one-sample rounds of form 3)
+int form_sb(int form) { return form == 1 ? 0 : (form == 2 ? 2 : 1); }
 // largest rows-per-workgroup x RNU*RMU_S of a batch (k_inscatterN staging)
 int sn_max_row_texels(const std::vector<atmo_params> &ps, int snCap) {
     int m = 0;
@@ -340,7 +341,7 @@ struct Plan {
     std::vector<JSeg> jseg;   // P_J tiles grouped by the per-set G
     int snCap = 4;     // k_inscatterN max mu rows per workgroup
     int snXcd = 1;     // k_inscatterN XCD-banded tile order
-    int snSb = 0;      // k_inscatterN: 0 direct gathers, > 0 LDS-staged rows (samples per round)
+    int snSb = 1;      // k_inscatterN: 0 direct gathers, > 0 LDS-staged rows (samples per round)
     bool tuned = false;
     void drop_graph() {
         if (gexec) (void)hipGraphExecDestroy(gexec);

@@ -953,12 +953,17 @@ __device__ __forceinline__ float sn_txs(const SetDesc &s, float nu, float mus, f
     if (LIN4)
         return fmaxf(fmaf(nu, en.tw.w, fmaf(0.2f, en.ir.y, mus)) * en.ir.x, 0.0f) * (1.0f / 1.2f) *
                (2.0f * half);
+    // Every sum of products is an explicit fmaf: left to contraction, the
+    // compiler fuses kb + kd as fma(k, d, kb) in one inlined copy and as
+    // fma(k, mub, kd) in another, and the paths of k_inscatterN must agree
+    // bit for bit.
     const float mub = fmaxf(mus, -0.1975f);
-    const float dmu = fmaf(nu, en.tw.w, -mus * en.ir.y) * en.ir.x;   // mu_s_i - mu_s
-    const float d = fmaxf(dmu + (mus - mub), -0.1975f - mub);
+    // d = mu_s_i - mu_s + (mu_s - mu_b), dmu = (nu t - mu_s (r_i - r)) / r_i
+    const float d = fmaxf(fmaf(fmaf(nu, en.tw.w, -mus * en.ir.y), en.ir.x, mus - mub), -0.1975f - mub);
     const float k = s.f_tMuS, kb = k * mub, kd = k * d;
+    const float kbd = fmaf(k, d, kb);   // kb + kd
     const bool near = fabsf(kd) < 0.5f;
-    const float arg = near ? kd * __builtin_amdgcn_rcpf(fmaf(kb, kb + kd, 1.0f)) : kb + kd;
+    const float arg = near ? kd * __builtin_amdgcn_rcpf(fmaf(kb, kbd, 1.0f)) : kbd;
     return fmaf(atanf(arg), half * (1.0f / 1.1f), near ? txb : 0.74f * half);
 }
 
@@ -970,7 +975,9 @@ __device__ __forceinline__ float sn_txs(const SetDesc &s, float nu, float mus, f
 // neighbours of B_i and lerps in x.  The direct form blends the two columns
 // it needs in the same order (sn_blend, sn_lerp), so both give the same bits.
 // `sb` samples are blended per LDS round (dynamic LDS: sb * rows * RNU*RMU_S
-// float4).
+// float4).  A workgroup covers rows of RNU*RMU_S <= 256 texels with one texel
+// per thread, or one row of more with 2-4 texels per thread: the staged form
+// has a path for each.
 template <bool STAGED, bool LIN4>   // LIN4: linear inscatter parameterisation (per-sample mu_s)
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_inscatterN(const SetDesc *__restrict__ sets,
                                                     const uint2 *__restrict__ tiles,
@@ -1117,59 +1124,143 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 }
                 acc[j][0] = a0; acc[j][1] = a1; acc[j][2] = a2;
             }
-        } else {
+        } else if (nmine >= 2) {
+            // Rows of more than 256 texels (one row per workgroup, 2-4 texels
+            // per thread): every slot reads the same sample entries, so (a)
+            // issues all 4 * NM table loads of a thread before blending any,
+            // and (b) interleaves the NM independent lookups of a sample.  The
+            // arithmetic of each texel-sample is that of the general path.
+            auto one_row = [&](auto nmc) {
+                constexpr int NM = decltype(nmc)::value;
+                for (int s0 = 0; s0 < nc; s0 += sb) {
+                    const int ns = min(sb, nc - s0);
+                    for (int k = 0; k < ns; ++k) {
+                        const float4 w = ent[0][s0 + k].w;
+                        const int4 base = ent[0][s0 + k].base;
+                        float3 t[NM][4];
+#pragma unroll
+                        for (int j = 0; j < NM; ++j) {
+                            // past the row's end: a valid address, the blend is not stored
+                            const unsigned xo = (unsigned)min((int)threadIdx.x + j * (int)blockDim.x, per - 1) * jb;
+                            t[j][0] = ld3((const gfloat4 *)(J + (unsigned)base.x + xo));
+                            t[j][1] = ld3((const gfloat4 *)(J + (unsigned)base.y + xo));
+                            t[j][2] = ld3((const gfloat4 *)(J + (unsigned)base.z + xo));
+                            t[j][3] = ld3((const gfloat4 *)(J + (unsigned)base.w + xo));
+                        }
+#pragma unroll
+                        for (int j = 0; j < NM; ++j) {
+                            const int e = threadIdx.x + j * (int)blockDim.x;
+                            const float3 b = sn_blend(w, t[j][0], t[j][1], t[j][2], t[j][3]);
+                            if (e < per) blend[(size_t)k * per + e] = make_float4(b.x, b.y, b.z, 0.f);
+                        }
+                    }
+                    __syncthreads();
+                    for (int k = 0; k < ns; ++k) {
+                        const SnEntry en = ent[0][s0 + k];
+                        const float4 *B = blend + (size_t)k * per;
+                        float3 v[NM];
+                        unsigned i0[NM];
+                        float ax[NM];
+#pragma unroll
+                        for (int j = 0; j < NM; ++j) {
+                            // slots past the row's end (kq 0) read in-row values and are never stored
+                            const unsigned kbase = (unsigned)((kq[j] & 0xffff) * s.RMUS);
+                            const float txs = __builtin_amdgcn_fmed3f(sn_txs<LIN4>(s, nuf[j], muSr[j], txb[j], en), 0.0f,
+                                                                      (float)(W4m2 + 1 - kbase));
+                            const unsigned is = min(__float2uint_rz(txs), W4m2 - kbase);
+                            ax[j] = txs - (float)is;
+                            i0[j] = kbase + is;
+                            v[j] = sn_lerp(ax[j], f3(B[i0[j]]), f3(B[i0[j] + 1]));
+                        }
+#pragma unroll
+                        for (int j = 0; j < NM; ++j) {
+                            if (fk[j] != 0.0f) {  // off-grid nu: blend with the next slice (k0 + 1 <= RNU - 1)
+                                unsigned j0 = i0[j] + (unsigned)s.RMUS;
+                                float bx = ax[j];
+                                if (j0 > W4m2) { j0 = W4m2; bx = 1.0f; }   // past the row's last column
+                                v[j] = lerp3(v[j], sn_lerp(bx, f3(B[j0]), f3(B[j0 + 1])), fk[j]);
+                            }
+                            acc[j][0] = fmaf(v[j].x, en.tw.x, acc[j][0]);
+                            acc[j][1] = fmaf(v[j].y, en.tw.y, acc[j][1]);
+                            acc[j][2] = fmaf(v[j].z, en.tw.z, acc[j][2]);
+                        }
+                    }
+                    __syncthreads();
+                }
+            };
+            if (nmine == 2) one_row(std::integral_constant<int, 2>{});
+            else if (nmine == 3) one_row(std::integral_constant<int, 3>{});
+            else one_row(std::integral_constant<int, MAXT>{});
+        } else if (nmine == 1) {
+            // Rows of at most 256 texels (one texel per thread): samples are
+            // blended and looked up in pairs, so a thread has 8 table loads and
+            // two independent lookups in flight; the accumulation keeps sample
+            // order, and each texel-sample's arithmetic is the general path's.
+            const int e = threadIdx.x;
+            const bool live = e < per;
+            const int rowj = kq[0] >> 16;   // 0 past the rows' end: valid addresses, nothing stored
+            const SnEntry *er = ent[rowj];
+            const unsigned xo = (unsigned)(live ? e - rowj * s.W4 : 0) * jb;
+            const unsigned kbase = (unsigned)((kq[0] & 0xffff) * s.RMUS);
+            const float txmax = (float)(W4m2 + 1 - kbase);
+            const unsigned ismax = W4m2 - kbase;
+            const float4 *Brow = blend + rowj * s.W4;
+            auto group = [&](int s0, int k, auto pc) {   // samples s0+k .. s0+k+P-1
+                constexpr int P = decltype(pc)::value;
+                float3 t[P][4];
+#pragma unroll
+                for (int q = 0; q < P; ++q) {
+                    const int4 base = er[s0 + k + q].base;
+                    t[q][0] = ld3((const gfloat4 *)(J + (unsigned)base.x + xo));
+                    t[q][1] = ld3((const gfloat4 *)(J + (unsigned)base.y + xo));
+                    t[q][2] = ld3((const gfloat4 *)(J + (unsigned)base.z + xo));
+                    t[q][3] = ld3((const gfloat4 *)(J + (unsigned)base.w + xo));
+                }
+#pragma unroll
+                for (int q = 0; q < P; ++q) {
+                    const float3 b = sn_blend(er[s0 + k + q].w, t[q][0], t[q][1], t[q][2], t[q][3]);
+                    if (live) blend[(size_t)(k + q) * per + e] = make_float4(b.x, b.y, b.z, 0.f);
+                }
+            };
+            auto lookup = [&](int s0, int k, auto pc) {
+                constexpr int P = decltype(pc)::value;
+                float3 v[P];
+                unsigned i0[P];
+                float ax[P];
+#pragma unroll
+                for (int q = 0; q < P; ++q) {
+                    const float txs = __builtin_amdgcn_fmed3f(
+                        sn_txs<LIN4>(s, nuf[0], muSr[0], txb[0], er[s0 + k + q]), 0.0f, txmax);
+                    const unsigned is = min(__float2uint_rz(txs), ismax);
+                    ax[q] = txs - (float)is;
+                    i0[q] = kbase + is;
+                    const float4 *B = Brow + (size_t)(k + q) * per;
+                    v[q] = sn_lerp(ax[q], f3(B[i0[q]]), f3(B[i0[q] + 1]));
+                }
+#pragma unroll
+                for (int q = 0; q < P; ++q) {
+                    if (fk[0] != 0.0f) {  // off-grid nu: blend with the next slice (k0 + 1 <= RNU - 1)
+                        const float4 *B = Brow + (size_t)(k + q) * per;
+                        unsigned j0 = i0[q] + (unsigned)s.RMUS;
+                        float bx = ax[q];
+                        if (j0 > W4m2) { j0 = W4m2; bx = 1.0f; }   // past the row's last column
+                        v[q] = lerp3(v[q], sn_lerp(bx, f3(B[j0]), f3(B[j0 + 1])), fk[0]);
+                    }
+                    const float4 tw = er[s0 + k + q].tw;
+                    acc[0][0] = fmaf(v[q].x, tw.x, acc[0][0]);
+                    acc[0][1] = fmaf(v[q].y, tw.y, acc[0][1]);
+                    acc[0][2] = fmaf(v[q].z, tw.z, acc[0][2]);
+                }
+            };
             for (int s0 = 0; s0 < nc; s0 += sb) {
                 const int ns = min(sb, nc - s0);
-                // (a) blended rows of samples s0 .. s0+ns-1: entry e = (row, x), as the texels
-                for (int k = 0; k < ns; ++k) {
-#pragma unroll
-                    for (int j = 0; j < MAXT; ++j) {
-                        if (j >= nmine) break;
-                        const int e = threadIdx.x + j * (int)blockDim.x;
-                        if (e >= per) break;
-                        const int rowj = kq[j] >> 16;
-                        const SnEntry &en = ent[rowj][s0 + k];
-                        const unsigned xo = (unsigned)(e - rowj * s.W4) * jb;
-                        const float3 b = sn_blend(en.w, ld3((const gfloat4 *)(J + (unsigned)en.base.x + xo)),
-                                                  ld3((const gfloat4 *)(J + (unsigned)en.base.y + xo)),
-                                                  ld3((const gfloat4 *)(J + (unsigned)en.base.z + xo)),
-                                                  ld3((const gfloat4 *)(J + (unsigned)en.base.w + xo)));
-                        blend[(size_t)k * per + e] = make_float4(b.x, b.y, b.z, 0.f);
-                    }
-                }
+                int k = 0;
+                for (; k + 1 < ns; k += 2) group(s0, k, std::integral_constant<int, 2>{});
+                if (k < ns) group(s0, k, std::integral_constant<int, 1>{});
                 __syncthreads();
-                // (b) per texel: x lerp of the blended row
-#pragma unroll
-                for (int j = 0; j < MAXT; ++j) {
-                    if (j >= nmine) break;
-                    const int e = threadIdx.x + j * (int)blockDim.x;
-                    if (e >= per) break;
-                    const SnEntry *er = ent[kq[j] >> 16];
-                    const unsigned kbase = (unsigned)((kq[j] & 0xffff) * s.RMUS);
-                    const float txmax = (float)(W4m2 + 1 - kbase);
-                    const unsigned ismax = W4m2 - kbase;
-                    const int rowbase = (kq[j] >> 16) * s.W4;
-                    float a0 = acc[j][0], a1 = acc[j][1], a2 = acc[j][2];
-                    for (int k = 0; k < ns; ++k) {
-                        const SnEntry &en = er[s0 + k];
-                        const float txs = __builtin_amdgcn_fmed3f(sn_txs<LIN4>(s, nuf[j], muSr[j], txb[j], en), 0.0f, txmax);
-                        const unsigned is = min(__float2uint_rz(txs), ismax);
-                        const float ax = txs - (float)is;
-                        const unsigned i0 = kbase + is;
-                        const float4 *B = blend + (size_t)k * per + rowbase;
-                        float3 v = sn_lerp(ax, f3(B[i0]), f3(B[i0 + 1]));
-                        if (fk[j] != 0.0f) {  // off-grid nu: blend with the next slice (k0 + 1 <= RNU - 1)
-                            unsigned j0 = i0 + (unsigned)s.RMUS;
-                            float bx = ax;
-                            if (j0 > W4m2) { j0 = W4m2; bx = 1.0f; }   // past the row's last column
-                            v = lerp3(v, sn_lerp(bx, f3(B[j0]), f3(B[j0 + 1])), fk[j]);
-                        }
-                        a0 = fmaf(v.x, en.tw.x, a0);
-                        a1 = fmaf(v.y, en.tw.y, a1);
-                        a2 = fmaf(v.z, en.tw.z, a2);
-                    }
-                    acc[j][0] = a0; acc[j][1] = a1; acc[j][2] = a2;
-                }
+                k = 0;
+                for (; k + 1 < ns; k += 2) lookup(s0, k, std::integral_constant<int, 2>{});
+                if (k < ns) lookup(s0, k, std::integral_constant<int, 1>{});
                 __syncthreads();
             }
         }

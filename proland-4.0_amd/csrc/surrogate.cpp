@@ -347,9 +347,10 @@ atmo_status atmo_surrogate_set_exact_orders(atmo_surrogate *s, int orders) {
     return ATMO_OK;
 }
 
-atmo_status atmo_surrogate_features(const atmo_params *p, const uint32_t *codes, int n, float *X) {
-    if (!p || !codes || !X || n <= 0) return fail(ATMO_ERR_INVALID_ARGUMENT, "bad arguments");
-    HIPCHK(code_rows(feat_set(*p), codes, n, X, nullptr));
+atmo_status atmo_surrogate_features(const atmo_params *p, const float *S_exact, const float *E_exact,
+                                    const uint32_t *codes, int n, float *X) {
+    if (!p || !S_exact || !E_exact || !codes || !X || n <= 0) return fail(ATMO_ERR_INVALID_ARGUMENT, "bad arguments");
+    HIPCHK(code_rows(feat_set(*p), codes, n, (const float4 *)S_exact, E_exact, X, nullptr));
     HIPCHK(hipDeviceSynchronize());
     return ATMO_OK;
 }

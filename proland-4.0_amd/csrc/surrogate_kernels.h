@@ -6,7 +6,7 @@
 namespace atmo {
 namespace sg {
 
-constexpr int kNIN = 11;   // 4 texel coordinates + 7 atmosphere features
+constexpr int kNIN = 14;   // 4 texel coordinates + 7 atmosphere features + 3 exact-part values
 constexpr int kXS = 16;    // row stride of the input features (floats)
 constexpr int kNH = 256;   // hidden width
 constexpr int kNL = 4;     // hidden layers
@@ -33,7 +33,7 @@ struct TargetNorm {
 
 // Weights of the fused inference kernel (device pointers), from prep_fused.
 struct FusedWeights {
-    const __bf16 *a1h, *a1l;  // [256][16] layer 1 (column 11: bias), bf16 hi / lo
+    const __bf16 *a1h, *a1l;  // [256][16] layer 1 (column 14: bias), bf16 hi / lo
     const __bf16 *wt;         // [3][256][256] layers 2..4 as [out][in]
     const float *b234;        // [3][256] biases of layers 2..4
     const __bf16 *a5h, *a5l;  // [4][256] layer 5 as [out][in], bf16 hi / lo
@@ -68,13 +68,12 @@ hipError_t to_bf16(const float *W, __bf16 *Wb, __bf16 *Wt, hipStream_t st);
 hipError_t make_rows(const FeatSet &fs, const float4 *S4, const float4 *S1, const float *E, const float *E1, int ns,
                      unsigned long long seed, float *X, float *Tg, const TargetNorm &tn,
                      hipStream_t st);
-hipError_t infer_rows(const FeatSet &fs, int row0, int n, float *X, hipStream_t st);
-hipError_t code_rows(const FeatSet &fs, const unsigned *codes, int n, float *X, hipStream_t st);
+// S1, E1: the exact tables of the set (the exact-part features)
+hipError_t code_rows(const FeatSet &fs, const unsigned *codes, int n, const float4 *S1, const float *E1, float *X,
+                     hipStream_t st);
 // out[6] (device, zeroed by the caller): sums and sums of squares of the rgb targets
 hipError_t target_stats(const float *Tg, size_t n, double *out, hipStream_t st);
 hipError_t norm_targets(float *Tg, size_t n, const TargetNorm &tn, hipStream_t st);
-hipError_t apply(int row0, int n, int n4, const float *Y, const TargetNorm &tn, float4 *S, float *E,
-                 hipStream_t st);
 hipError_t prep_fused(const float *W1, const float *b1, const float *W5, const float *b2, const float *b3,
                       const float *b4, __bf16 *a1h, __bf16 *a1l, __bf16 *a5h, __bf16 *a5l, float *b234,
                       hipStream_t st);

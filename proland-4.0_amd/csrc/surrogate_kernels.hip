@@ -1,6 +1,6 @@
 // surrogate_kernels.hip — kernels of the learned multiple-scattering surrogate.
 //
-// MLP 11 -> 256 -> 256 -> 256 -> 256 -> 4, SiLU on the hidden layers.  The
+// MLP 14 -> 256 -> 256 -> 256 -> 256 -> 4, SiLU on the hidden layers.  The
 // three 256x256 layers (97 % of the FLOPs) run on the BF16 matrix cores
 // (v_mfma_f32_32x32x16_bf16, FP32 accumulation) in one LDS-tiled GEMM kernel
 // with transposable operands and fused epilogues; the thin first and last
@@ -120,8 +120,8 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A,
 }
 
 // ---------------------------------------------------------------------------
-// Layer 1 (11 -> 256), FP32: Z1 = X W1 + b1, H1 = SiLU(Z1).  X rows are 16
-// floats (11 used).  One thread per (row, column).
+// Layer 1 (14 -> 256), FP32: Z1 = X W1 + b1, H1 = SiLU(Z1).  X rows are 16
+// floats (14 used).  One thread per (row, column).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_layer1_fwd(const float *__restrict__ X,
                                                     const float *__restrict__ W1,
@@ -309,9 +309,14 @@ __global__ __launch_bounds__(256) void k_to_bf16(const float *__restrict__ W, __
 constexpr float kHorizonGap = 0.25f;
 
 // Resolution-independent texel coordinates in [-1, 1] (the arguments of
-// getMuMuSNu, common.glsl:175-201) plus the 7 normalised atmosphere features.
+// getMuMuSNu, common.glsl:175-201), the 7 normalised atmosphere features, and
+// the exactly computed orders' value at the texel (ex: S.rgb of an inscatter
+// texel, E of an irradiance texel) on a log scale: 0.25 (log(v + 1e-6) + 5),
+// about -2 .. 1 over the tables' range.  The higher orders follow the lower
+// ones closely (where the view ray meets the ground, where the sun sets), so
+// the network learns them relative to what the exact part already resolves.
 // E texels use the irradiance coordinates in slots 0 and 2 and -2 in 1 and 3.
-__device__ __forceinline__ void texel_features(const FeatSet &fs, unsigned code, float *x) {
+__device__ __forceinline__ void texel_features(const FeatSet &fs, unsigned code, const float *ex, float *x) {
     if (code & 0x80000000u) {
         const unsigned t = code & 0x7fffffffu;
         const int i = t % fs.SW, j = t / fs.SW;
@@ -358,6 +363,24 @@ __device__ __forceinline__ void texel_features(const FeatSet &fs, unsigned code,
     }
 #pragma unroll
     for (int k = 0; k < 7; ++k) x[4 + k] = fs.f[k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x[11 + c] = 0.25f * (__logf(fmaxf(ex[c], 0.f) + 1e-6f) + 5.0f);
+}
+
+// The exact part's value at a texel (S.rgb, or E of an irradiance texel).
+__device__ __forceinline__ void exact_value(const FeatSet &fs, unsigned code, const float4 *S1, const float *E1,
+                                            float *ex) {
+    if (code & 0x80000000u) {
+        const unsigned e = code & 0x7fffffffu;
+        ex[0] = E1[3 * e];
+        ex[1] = E1[3 * e + 1];
+        ex[2] = E1[3 * e + 2];
+    } else {
+        const float4 v = S1[code];
+        ex[0] = v.x;
+        ex[1] = v.y;
+        ex[2] = v.z;
+    }
 }
 
 __device__ __forceinline__ unsigned hash32(unsigned long long z) {
@@ -381,18 +404,20 @@ __global__ __launch_bounds__(256) void k_make_rows(FeatSet fs, const float4 *__r
     const int nE = fs.SW * fs.SH;
     if (t >= ns + nE) return;
     unsigned code;
-    float v[3];
+    float v[3], ex[3];
     if (t < ns) {
         code = hash32(seed * 0x100000001ull + t) % (unsigned)fs.n4;
         const float4 a = S4[code], b = S1[code];
         v[0] = a.x - b.x; v[1] = a.y - b.y; v[2] = a.z - b.z;
+        ex[0] = b.x; ex[1] = b.y; ex[2] = b.z;
     } else {
         const int e = t - ns;
         code = 0x80000000u | (unsigned)e;
         v[0] = E[3 * e] - E1[3 * e]; v[1] = E[3 * e + 1] - E1[3 * e + 1]; v[2] = E[3 * e + 2] - E1[3 * e + 2];
+        ex[0] = E1[3 * e]; ex[1] = E1[3 * e + 1]; ex[2] = E1[3 * e + 2];
     }
     float x[kNIN];
-    texel_features(fs, code, x);
+    texel_features(fs, code, ex, x);
 #pragma unroll
     for (int i = 0; i < kNIN; ++i) X[(size_t)t * kXS + i] = x[i];
 #pragma unroll
@@ -402,29 +427,15 @@ __global__ __launch_bounds__(256) void k_make_rows(FeatSet fs, const float4 *__r
     Tg[(size_t)t * kNOUT + 3] = 0.f;
 }
 
-// Inference rows of one set: rows [row0, row0 + n) of the set's S texels
-// (code < n4) and then its E texels.
-__global__ __launch_bounds__(256) void k_infer_rows(FeatSet fs, int row0, int n,
-                                                    float *__restrict__ X) {
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= n) return;
-    const int r = row0 + t;
-    const unsigned code = r < fs.n4 ? (unsigned)r : (0x80000000u | (unsigned)(r - fs.n4));
-    float x[kNIN];
-    texel_features(fs, code, x);
-#pragma unroll
-    for (int i = 0; i < kNIN; ++i) X[(size_t)t * kXS + i] = x[i];
-#pragma unroll
-    for (int i = kNIN; i < kXS; ++i) X[(size_t)t * kXS + i] = 0.f;
-}
-
 // Rows for caller-given texel codes.
 __global__ __launch_bounds__(256) void k_code_rows(FeatSet fs, const unsigned *__restrict__ codes,
-                                                   int n, float *__restrict__ X) {
+                                                   int n, const float4 *__restrict__ S1,
+                                                   const float *__restrict__ E1, float *__restrict__ X) {
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= n) return;
-    float x[kNIN];
-    texel_features(fs, codes[t], x);
+    float ex[3], x[kNIN];
+    exact_value(fs, codes[t], S1, E1, ex);
+    texel_features(fs, codes[t], ex, x);
 #pragma unroll
     for (int i = 0; i < kNIN; ++i) X[(size_t)t * kXS + i] = x[i];
 #pragma unroll
@@ -465,26 +476,6 @@ __global__ __launch_bounds__(256) void k_norm_targets(float *__restrict__ Tg, si
     for (int c = 0; c < 3; ++c) Tg[t * kNOUT + c] = (Tg[t * kNOUT + c] - tn.mu[c]) / tn.sd[c];
 }
 
-// Adds the predicted higher orders to the exact part (S.rgb and E).  Y rows
-// follow k_infer_rows.
-__global__ __launch_bounds__(256) void k_apply(int row0, int n, int n4, const float *__restrict__ Y,
-                                               TargetNorm tn, float4 *__restrict__ S,
-                                               float *__restrict__ E) {
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= n) return;
-    const int r = row0 + t;
-    float v[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) v[c] = fmaxf(__expf(Y[(size_t)t * kNOUT + c] * tn.sd[c] + tn.mu[c]) - tn.eps, 0.f);
-    if (r < n4) {
-        float4 s = S[r];
-        s.x += v[0]; s.y += v[1]; s.z += v[2];
-        S[r] = s;
-    } else {
-        const int e = r - n4;
-        E[3 * e] += v[0]; E[3 * e + 1] += v[1]; E[3 * e + 2] += v[2];
-    }
-}
 
 // Random training batch: rows idx = hash(seed, b) mod n copied from the dataset.
 __global__ __launch_bounds__(256) void k_gather(const float *__restrict__ Xa, const float *__restrict__ Ta,
@@ -548,7 +539,7 @@ __global__ __launch_bounds__(256) void k_absmax(const float *__restrict__ a, siz
 // (rows padded to 528 B, conflict-free 16-byte LDS reads) per layer.
 // Layers 1 and 5 use the same MFMA with FP32 weights split into bf16 hi + lo
 // parts (x W ~ xh Wh + xh Wl + xl Wh for layer 1, whose bias rides in input
-// column 11; H4 is bf16 already, so layer 5 needs Wh + Wl only): relative
+// column 14; H4 is bf16 already, so layer 5 needs Wh + Wl only): relative
 // error ~1e-5, well inside the bf16 rounding of the next activation.
 // ---------------------------------------------------------------------------
 constexpr int kRowPad = 264;   // bf16 per staged weight row (512 B + 16 B pad)
@@ -693,8 +684,9 @@ __global__ __launch_bounds__(64 * NW) void k_mlp_infer(const InferSet *__restric
     for (int rb = 0; rb < RBW; ++rb) {
         const int r = min(rw + 32 * rb + c, n - 1) + row0;
         const unsigned code = r < fs.n4 ? (unsigned)r : (0x80000000u | (unsigned)(r - fs.n4));
-        float x[kNIN];
-        texel_features(fs, code, x);
+        float ex[3], x[kNIN];
+        exact_value(fs, code, is.S, is.E, ex);   // the exact part, before this kernel adds to it
+        texel_features(fs, code, ex, x);
         float xe[16];
 #pragma unroll
         for (int k = 0; k < 16; ++k) xe[k] = k < kNIN ? x[k] : (k == kNIN ? 1.0f : 0.0f);
@@ -783,7 +775,7 @@ __global__ __launch_bounds__(64 * NW) void k_mlp_infer(const InferSet *__restric
 }
 
 // Weights of the fused kernel from the FP32 parameters: layer 1 (+ bias in
-// column 11) and layer 5 split into bf16 hi/lo, the hidden biases packed.
+// column 14) and layer 5 split into bf16 hi/lo, the hidden biases packed.
 __global__ __launch_bounds__(256) void k_prep_fused(const float *__restrict__ W1, const float *__restrict__ b1,
                                                     const float *__restrict__ W5, const float *__restrict__ b2,
                                                     const float *__restrict__ b3, const float *__restrict__ b4,
@@ -872,12 +864,9 @@ hipError_t make_rows(const FeatSet &fs, const float4 *S4, const float4 *S1, cons
     hipLaunchKernelGGL(k_make_rows, dim3(blocks(n)), dim3(256), 0, st, fs, S4, S1, E, E1, ns, seed, X, Tg, tn);
     return hipGetLastError();
 }
-hipError_t infer_rows(const FeatSet &fs, int row0, int n, float *X, hipStream_t st) {
-    hipLaunchKernelGGL(k_infer_rows, dim3(blocks(n)), dim3(256), 0, st, fs, row0, n, X);
-    return hipGetLastError();
-}
-hipError_t code_rows(const FeatSet &fs, const unsigned *codes, int n, float *X, hipStream_t st) {
-    hipLaunchKernelGGL(k_code_rows, dim3(blocks(n)), dim3(256), 0, st, fs, codes, n, X);
+hipError_t code_rows(const FeatSet &fs, const unsigned *codes, int n, const float4 *S1, const float *E1, float *X,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(k_code_rows, dim3(blocks(n)), dim3(256), 0, st, fs, codes, n, S1, E1, X);
     return hipGetLastError();
 }
 hipError_t target_stats(const float *Tg, size_t n, double *out, hipStream_t st) {
@@ -886,11 +875,6 @@ hipError_t target_stats(const float *Tg, size_t n, double *out, hipStream_t st) 
 }
 hipError_t norm_targets(float *Tg, size_t n, const TargetNorm &tn, hipStream_t st) {
     hipLaunchKernelGGL(k_norm_targets, dim3(blocks(n)), dim3(256), 0, st, Tg, n, tn);
-    return hipGetLastError();
-}
-hipError_t apply(int row0, int n, int n4, const float *Y, const TargetNorm &tn, float4 *S, float *E,
-                 hipStream_t st) {
-    hipLaunchKernelGGL(k_apply, dim3(blocks(n)), dim3(256), 0, st, row0, n, n4, Y, tn, S, E);
     return hipGetLastError();
 }
 hipError_t gather(const float *Xa, const float *Ta, size_t n, int B, unsigned long long seed, float *X,

@@ -7,7 +7,7 @@ import numpy as np
 from . import lib as L
 from .params import AtmoOptions, AtmoParams, options as default_options, params_array
 
-NIN, XS, NH, NOUT = 11, 16, 256, 4
+NIN, XS, NH, NOUT = 14, 16, 256, 4
 
 
 class Report(ctypes.Structure):
@@ -40,7 +40,7 @@ SIGNATURES = {
                                             ctypes.c_float, ctypes.c_uint64, _fp]),
     "atmo_surrogate_loss_grad": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _fp, _fp]),
     "atmo_surrogate_predict": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
-    "atmo_surrogate_features": (ctypes.c_int, [ctypes.POINTER(AtmoParams), _vp, ctypes.c_int, _vp]),
+    "atmo_surrogate_features": (ctypes.c_int, [ctypes.POINTER(AtmoParams), _vp, _vp, _vp, ctypes.c_int, _vp]),
     "atmo_surrogate_precompute": (ctypes.c_int, [_vp, ctypes.POINTER(AtmoParams), ctypes.c_int,
                                                  ctypes.POINTER(AtmoOptions),
                                                  ctypes.POINTER(L.AtmoTables), ctypes.c_float,
@@ -166,11 +166,17 @@ class Surrogate:
         return outs, rep.as_dict()
 
 
-def features(p, codes):
-    """Network input rows (n, 16) for texel codes of set p (device tensors)."""
+def features(p, codes, S_exact, E_exact):
+    """Network input rows (n, 16) for texel codes of set p (device tensors).
+    S_exact, E_exact: the set's device tables of the exactly computed orders
+    (float32, inscatter with 4 and irradiance with 3 components per texel)."""
     import torch
+    for t, c in ((S_exact, 4), (E_exact, 3)):
+        if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and t.shape[-1] == c):
+            raise ValueError("exact tables: contiguous float32 device tensors with %d components" % c)
     codes = torch.as_tensor(np.asarray(codes, dtype=np.uint32).view(np.int32)).cuda()
     X = torch.empty((codes.shape[0], XS), dtype=torch.float32, device="cuda")
-    L.check(lib().atmo_surrogate_features(p, ctypes.c_void_p(codes.data_ptr()), codes.shape[0],
+    L.check(lib().atmo_surrogate_features(p, ctypes.c_void_p(S_exact.data_ptr()), ctypes.c_void_p(E_exact.data_ptr()),
+                                          ctypes.c_void_p(codes.data_ptr()), codes.shape[0],
                                           ctypes.c_void_p(X.data_ptr())))
     return X

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     assert snames == set(SG.SIGNATURES)
     for n in snames:
         assert hasattr(lib, n), n
-    assert SG.param_count() == 201476
+    assert SG.param_count() == 202244
     from prolandatmo import consumer as C
     cnames = declared("atmo_consumer.h")
     assert cnames == set(C.SIGNATURES) and len(cnames) == 5

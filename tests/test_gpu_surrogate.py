@@ -26,7 +26,7 @@ def sur():
 def rows(B, seed):
     rng = np.random.default_rng(seed)
     X = np.zeros((B, 16), dtype=np.float32)
-    X[:, :11] = rng.uniform(-1, 1, (B, 11))
+    X[:, :SG.NIN] = rng.uniform(-1, 1, (B, SG.NIN))
     T = rng.normal(0, 1, (B, 4)).astype(np.float32)
     return X, T
 
@@ -81,7 +81,14 @@ def test_features_of_texel_codes():
     l, y, x = 3, p.RES_MU // 2 + 5, 2 * p.RES_MU_S + 7
     code = (l * p.RES_MU + y) * W4 + x
     e = 0x80000000 | (2 * p.SKY_W + 9)
-    X = SG.features(p, [code, e]).cpu().numpy()
+    # exact-part tables: positive values over the tables' range, and a zero texel
+    g = np.random.default_rng(5)
+    n4 = p.RES_R * p.RES_MU * W4
+    S1 = (10.0 ** g.uniform(-7, 0, (n4, 4))).astype(np.float32)
+    E1 = (10.0 ** g.uniform(-7, 0, (p.SKY_W * p.SKY_H, 3))).astype(np.float32)
+    S1[code, 1] = 0.0
+    S1d, E1d = torch.from_numpy(S1).cuda(), torch.from_numpy(E1).cuda()
+    X = SG.features(p, [code, e], S1d, E1d).cpu().numpy()
     half = p.RES_MU // 2
     x1 = 0.25 + 0.75 * _log_ray(p, l, y)           # sky half of the mu rows: [0.25, 1]
     assert X[0, :4] == pytest.approx([2 * l / (p.RES_R - 1) - 1, x1,
@@ -89,11 +96,16 @@ def test_features_of_texel_codes():
     assert X[1, :4] == pytest.approx([2 * 2 / (p.SKY_H - 1) - 1, -2, 2 * 9 / (p.SKY_W - 1) - 1, -2], abs=1e-6)
     for yg in (0, 3, half - 2, half - 1):          # ground half: [-1, -0.25]
         for lg in (0, l, p.RES_R - 1):
-            Xg = SG.features(p, [(lg * p.RES_MU + yg) * W4 + x]).cpu().numpy()
+            Xg = SG.features(p, [(lg * p.RES_MU + yg) * W4 + x], S1d, E1d).cpu().numpy()
             assert Xg[0, 1] == pytest.approx(-0.25 - 0.75 * _log_ray(p, lg, yg), abs=1e-5)
     assert X[0, 4] == pytest.approx((p.Rg - 5000) / 2000, abs=1e-6)
     assert X[0, 10] == pytest.approx((p.mieG - 0.75) / 0.15, abs=1e-5)
-    assert np.all(X[:, 11:] == 0)
+    # the exact part on a log scale: 0.25 (log(v + 1e-6) + 5)
+    logv = lambda v: 0.25 * (np.log(v.astype(np.float64) + 1e-6) + 5.0)
+    assert X[0, 11:14] == pytest.approx(logv(S1[code, :3]), abs=2e-6)
+    assert X[0, 12] == pytest.approx(0.25 * (np.log(1e-6) + 5.0), abs=2e-6)
+    assert X[1, 11:14] == pytest.approx(logv(E1[2 * p.SKY_W + 9]), abs=2e-6)
+    assert np.all(X[:, 14:] == 0)
 
 
 def small_sets(n, seed=SEED_BATCH):
@@ -188,7 +200,9 @@ def test_fused_inference_matches_the_network(sur):
             nE = p.SKY_W * p.SKY_H
             codes = np.concatenate([np.arange(n4, dtype=np.uint32),
                                     np.uint32(0x80000000) | np.arange(nE, dtype=np.uint32)])
-            Y = sur.predict(SG.features(p, codes)).double().cpu().numpy()
+            S1d = torch.from_numpy(rS.astype(np.float32)).cuda().reshape(-1, 4).contiguous()
+            E1d = torch.from_numpy(rE.astype(np.float32)).cuda().reshape(-1, 3).contiguous()
+            Y = sur.predict(SG.features(p, codes, S1d, E1d)).double().cpu().numpy()
             v = np.maximum(np.exp(Y[:, :3] * sd.astype(np.float64) + mu) - eps, 0.0)
             S = S.double().cpu().numpy().reshape(-1, 4)
             vS = S[:, :3] - rS.reshape(-1, 4)[:, :3]

@@ -88,7 +88,7 @@ def test_training_pass_at_batch_65536_against_the_oracle():
     B = 65536
     rng = np.random.default_rng(0x5EED0004)
     X = np.zeros((B, 16), dtype=np.float32)
-    X[:, :11] = rng.uniform(-1, 1, (B, 11))
+    X[:, :SG.NIN] = rng.uniform(-1, 1, (B, SG.NIN))
     T = rng.normal(0, 1, (B, 4)).astype(np.float32)
     loss, g = s.loss_grad(torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda())
     lo, go = M.loss_grad(s.get_params().astype(np.float64), X, T, emulate_bf16=True)
