@@ -10,15 +10,16 @@
  *
  * Network: 14 inputs (4 texel coordinates in [-1, 1], 7 normalised
  * atmosphere features, and the log-scaled rgb of the exactly computed orders
- * at the texel) -> 4 x 256 SiLU -> 4 outputs (log-scaled rgb of the
+ * at the texel) -> 3 x 256 SiLU -> 4 outputs (log-scaled rgb of the
  * contribution of the orders above the exactly computed ones, standardised;
  * the 4th output is trained to 0 because those orders add nothing to alpha).
- * The three 256x256 layers run in BF16 on the matrix cores with FP32
+ * The two 256x256 layers run in BF16 on the matrix cores with FP32
  * accumulation; master weights, Adam state, the first and last layers are
  * FP32.
  *
- * Parameter vector layout (FP32): W1[14][256], b1[256], then for l = 2..4
- * Wl[256][256] (row = input), bl[256], then W5[256][4], b5[4].
+ * Parameter vector layout (FP32): W1[14][256], b1[256], then for l = 2..3
+ * Wl[256][256] (row = input), bl[256], then the output layer W5[256][4],
+ * b5[4] (named layer 5 as in a four-hidden-layer network).
  */
 #ifndef ATMO_SURROGATE_H_
 #define ATMO_SURROGATE_H_
@@ -35,7 +36,7 @@ typedef struct atmo_surrogate atmo_surrogate;
 ATMO_API atmo_status atmo_surrogate_create(atmo_surrogate **out, int device, uint64_t seed);
 ATMO_API atmo_status atmo_surrogate_destroy(atmo_surrogate *s);
 
-/* Number of FP32 parameters (202244) and host copies of them. */
+/* Number of FP32 parameters (136452) and host copies of them. */
 ATMO_API size_t atmo_surrogate_param_count(void);
 ATMO_API atmo_status atmo_surrogate_get_params(atmo_surrogate *s, float *host, size_t n);
 ATMO_API atmo_status atmo_surrogate_set_params(atmo_surrogate *s, const float *host, size_t n);

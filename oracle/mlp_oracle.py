@@ -10,13 +10,16 @@ kernels store them, so the two agree up to accumulation order.
 import numpy as np
 
 NIN, XS, NH, NOUT = 14, 16, 256, 4
+NL = 3   # hidden layers (H1 .. H_NL); layers 2 .. NL are 256x256
 
 
 def layout():
     out, off = {}, 0
-    for name, shape in (("W1", (NIN, NH)), ("b1", (NH,)), ("W2", (NH, NH)), ("b2", (NH,)),
-                        ("W3", (NH, NH)), ("b3", (NH,)), ("W4", (NH, NH)), ("b4", (NH,)),
-                        ("W5", (NH, NOUT)), ("b5", (NOUT,))):
+    blocks = [("W1", (NIN, NH)), ("b1", (NH,))]
+    for L in range(2, NL + 1):   # the hidden 256x256 layers
+        blocks += [("W%d" % L, (NH, NH)), ("b%d" % L, (NH,))]
+    blocks += [("W5", (NH, NOUT)), ("b5", (NOUT,))]   # the output layer ("layer 5" for any NL)
+    for name, shape in blocks:
         out[name] = (off, shape)
         off += int(np.prod(shape))
     return out, off
@@ -61,7 +64,7 @@ def forward(p, X, emulate_bf16=True):
     X = np.asarray(X, dtype=np.float64)[:, :NIN]
     Z = [X @ w["W1"] + w["b1"]]
     H = [r(silu(Z[0]))]
-    for L in (2, 3, 4):
+    for L in range(2, NL + 1):
         Z.append(H[-1] @ r(w["W%d" % L]) + w["b%d" % L])
         H.append(r(silu(Z[-1])))
     Y = H[-1] @ w["W5"] + w["b5"]
@@ -76,9 +79,9 @@ def loss_grad(p, X, T, emulate_bf16=True):
     d = Y - T
     loss = float(np.sum(d * d) / (B * NOUT))
     dY = 2.0 * d / (B * NOUT)
-    g = {"W5": H[3].T @ dY, "b5": dY.sum(0)}
-    dZ = r((dY @ w["W5"].T) * dsilu(Z[3]))
-    for L in (4, 3, 2):
+    g = {"W5": H[NL - 1].T @ dY, "b5": dY.sum(0)}
+    dZ = r((dY @ w["W5"].T) * dsilu(Z[NL - 1]))
+    for L in range(NL, 1, -1):
         g["W%d" % L] = H[L - 2].T @ dZ
         g["b%d" % L] = dZ.sum(0)
         dH = dZ @ r(w["W%d" % L]).T

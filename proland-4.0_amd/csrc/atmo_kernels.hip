@@ -978,8 +978,12 @@ __device__ __forceinline__ float sn_txs(const SetDesc &s, float nu, float mus, f
 // float4).  A workgroup covers rows of RNU*RMU_S <= 256 texels with one texel
 // per thread, or one row of more with 2-4 texels per thread: the staged form
 // has a path for each.
+// Three waves per SIMD (up to 168 VGPRs): at four (128 VGPRs) the staged form
+// spills 116 bytes per lane, whose traffic shows in the kernel's HBM counters,
+// and both builds run at the same speed (2.31 and 2.33 ms per launch at
+// 64x256x64x16), the loads of a thread's texels being in flight together.
 template <bool STAGED, bool LIN4>   // LIN4: linear inscatter parameterisation (per-sample mu_s)
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_inscatterN(const SetDesc *__restrict__ sets,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_inscatterN(const SetDesc *__restrict__ sets,
                                                     const uint2 *__restrict__ tiles,
                                                     int nsamp, int rpw_cap, int sb) {
     __shared__ SnEntry ent[kNRows][kNChunk];   // exactly 16 KiB: two workgroups of 64 KiB

@@ -51,10 +51,10 @@ atmo_status fail(atmo_status st, const char *fmt, ...) {
 constexpr size_t OFF_W1 = 0, OFF_B1 = kNIN * kNH;
 constexpr size_t OFF_L0 = OFF_B1 + kNH;                 // W2
 constexpr size_t LAYER_SZ = (size_t)kNH * kNH + kNH;
-constexpr size_t OFF_W5 = OFF_L0 + 3 * LAYER_SZ;
+constexpr size_t OFF_W5 = OFF_L0 + kNG * LAYER_SZ;   // the output layer
 constexpr size_t OFF_B5 = OFF_W5 + (size_t)kNH * kNOUT;
-constexpr size_t NPARAM = OFF_B5 + kNOUT;               // 201476
-constexpr size_t offW(int L) { return OFF_L0 + (size_t)(L - 2) * LAYER_SZ; }   // L = 2..4
+constexpr size_t NPARAM = OFF_B5 + kNOUT;               // 136452 for kNL = 3
+constexpr size_t offW(int L) { return OFF_L0 + (size_t)(L - 2) * LAYER_SZ; }   // L = 2..kNL
 constexpr size_t offB(int L) { return offW(L) + (size_t)kNH * kNH; }
 
 struct Buf {
@@ -111,7 +111,7 @@ struct Tabs {  // device tables of one set in .raw layouts
 struct atmo_surrogate {
     int device = 0;
     hipStream_t st = nullptr;
-    Buf p, g, m, v, wb, wt;   // FP32 params/grads/Adam; bf16 W and W^T of layers 2..4
+    Buf p, g, m, v, wb, wt;   // FP32 params/grads/Adam; bf16 W and W^T of layers 2..kNL
     Buf fz;                   // fused-inference weights (prep_fused)
     Buf iset;                 // InferSet array of the last fused launch
     TargetNorm tn{{0, 0, 0}, {1, 1, 1}, 1e-6f};
@@ -131,8 +131,8 @@ struct atmo_surrogate {
     }
     __bf16 *Wb(int L) { return wb.as<__bf16>() + (size_t)(L - 2) * kNH * kNH; }
     __bf16 *Wt(int L) { return wt.as<__bf16>() + (size_t)(L - 2) * kNH * kNH; }
-    float *Zl(int i) { return Z.as<float>() + (size_t)i * zcap * kNH; }      // i = 0..3: Z1..Z4
-    __bf16 *Hl(int i) { return H.as<__bf16>() + (size_t)i * cap * kNH; }     // H1..H4
+    float *Zl(int i) { return Z.as<float>() + (size_t)i * zcap * kNH; }      // i = 0..kNL-1: Z1..Z_kNL
+    __bf16 *Hl(int i) { return H.as<__bf16>() + (size_t)i * cap * kNH; }     // H1..H_kNL
 };
 
 namespace {
@@ -142,13 +142,13 @@ atmo_status ensure_cap(atmo_surrogate *s, int rows, bool train) {
         const int c = rows;
         HIPCHK(s->X.ensure((size_t)c * kXS * 4));
         HIPCHK(s->T.ensure((size_t)c * kNOUT * 4));
-        HIPCHK(s->H.ensure((size_t)4 * c * kNH * 2));
+        HIPCHK(s->H.ensure((size_t)kNL * c * kNH * 2));
         HIPCHK(s->Y.ensure((size_t)c * kNOUT * 4));
         s->cap = c;
     }
     if (train && rows > s->zcap) {
         const int c = rows;
-        HIPCHK(s->Z.ensure((size_t)4 * c * kNH * 4));
+        HIPCHK(s->Z.ensure((size_t)kNL * c * kNH * 4));
         HIPCHK(s->dZa.ensure((size_t)c * kNH * 2));
         HIPCHK(s->dZb.ensure((size_t)c * kNH * 2));
         HIPCHK(s->dZ1.ensure((size_t)c * kNH * 4));
@@ -158,27 +158,28 @@ atmo_status ensure_cap(atmo_surrogate *s, int rows, bool train) {
     return ATMO_OK;
 }
 
-// fused-inference weight buffer: a1h, a1l [256][16], a5h, a5l [4][256] (bf16), b234 [3][256] (FP32)
+// fused-inference weight buffer: a1h, a1l [256][16], a5h, a5l [4][256] (bf16), hidden biases
+// [kNG][256] (FP32)
 constexpr size_t FZ_A1H = 0, FZ_A1L = FZ_A1H + kNH * 16 * 2, FZ_A5H = FZ_A1L + kNH * 16 * 2,
-                 FZ_A5L = FZ_A5H + kNOUT * kNH * 2, FZ_B234 = FZ_A5L + kNOUT * kNH * 2,
-                 FZ_BYTES = FZ_B234 + 3 * kNH * 4;
+                 FZ_A5L = FZ_A5H + kNOUT * kNH * 2, FZ_BH = FZ_A5L + kNOUT * kNH * 2,
+                 FZ_BYTES = FZ_BH + kNG * kNH * 4;
 
 FusedWeights fused_weights(atmo_surrogate *s) {
     char *z = (char *)s->fz.p;
     const float *P = s->p.as<float>();
     return FusedWeights{(const __bf16 *)(z + FZ_A1H), (const __bf16 *)(z + FZ_A1L), s->Wt(2),
-                        (const float *)(z + FZ_B234), (const __bf16 *)(z + FZ_A5H), (const __bf16 *)(z + FZ_A5L),
+                        (const float *)(z + FZ_BH), (const __bf16 *)(z + FZ_A5H), (const __bf16 *)(z + FZ_A5L),
                         P + OFF_B5};
 }
 
 atmo_status refresh_bf16(atmo_surrogate *s) {
-    for (int L = 2; L <= 4; ++L)
+    for (int L = 2; L <= kNL; ++L)
         HIPCHK(to_bf16(s->p.as<float>() + offW(L), s->Wb(L), s->Wt(L), s->st));
     const float *P = s->p.as<float>();
     char *z = (char *)s->fz.p;
-    HIPCHK(prep_fused(P + OFF_W1, P + OFF_B1, P + OFF_W5, P + offB(2), P + offB(3), P + offB(4),
-                      (__bf16 *)(z + FZ_A1H), (__bf16 *)(z + FZ_A1L), (__bf16 *)(z + FZ_A5H),
-                      (__bf16 *)(z + FZ_A5L), (float *)(z + FZ_B234), s->st));
+    HIPCHK(prep_fused(P + OFF_W1, P + OFF_B1, P + OFF_W5, P + offB(2), LAYER_SZ, (__bf16 *)(z + FZ_A1H),
+                      (__bf16 *)(z + FZ_A1L), (__bf16 *)(z + FZ_A5H), (__bf16 *)(z + FZ_A5L),
+                      (float *)(z + FZ_BH), s->st));
     return ATMO_OK;
 }
 
@@ -188,12 +189,12 @@ atmo_status forward(atmo_surrogate *s, int rows, bool train, float *loss_slot, f
     const float *P = s->p.as<float>();
     HIPCHK(layer1_fwd(s->X.as<float>(), P + OFF_W1, P + OFF_B1, rows, train ? s->Zl(0) : nullptr,
                       s->Hl(0), s->st));
-    for (int L = 2; L <= 4; ++L) {
+    for (int L = 2; L <= kNL; ++L) {
         GemmEpi ep{P + offB(L), nullptr, train ? s->Zl(L - 1) : nullptr, s->Hl(L - 1)};
         HIPCHK(gemm(false, true, train ? EPI_FWD : EPI_FWD_INF, s->Hl(L - 2), s->Wt(L), rows, kNH, kNH,
                     1, ep, s->st));
     }
-    HIPCHK(layer5(s->Hl(3), P + OFF_W5, P + OFF_B5, s->T.as<float>(), rows, train ? 0 : 1,
+    HIPCHK(layer5(s->Hl(kNL - 1), P + OFF_W5, P + OFF_B5, s->T.as<float>(), rows, train ? 0 : 1,
                   Yout, s->dY.as<float>(), loss_slot, s->st));
     return ATMO_OK;
 }
@@ -203,12 +204,12 @@ atmo_status backward(atmo_surrogate *s, int rows) {
     float *G = s->g.as<float>();
     HIPCHK(hipMemsetAsync(G, 0, NPARAM * 4, s->st));
     __bf16 *cur = s->dZa.as<__bf16>(), *nxt = s->dZb.as<__bf16>();
-    HIPCHK(layer5_bwd(s->Hl(3), s->Zl(3), P + OFF_W5, s->dY.as<float>(), rows, G + OFF_W5, G + OFF_B5,
+    HIPCHK(layer5_bwd(s->Hl(kNL - 1), s->Zl(kNL - 1), P + OFF_W5, s->dY.as<float>(), rows, G + OFF_W5, G + OFF_B5,
                       cur, s->st));
     const int nsplit = std::max(1, std::min(64, rows / 1024));
     const int ns = (rows / nsplit) % 32 == 0 ? nsplit : 1;
     HIPCHK(s->part.ensure((size_t)ns * kNH * kNH * 4));
-    for (int L = 4; L >= 2; --L) {
+    for (int L = kNL; L >= 2; --L) {
         GemmEpi ew{nullptr, nullptr, s->part.as<float>(), nullptr};
         HIPCHK(gemm(true, false, EPI_PARTIAL, s->Hl(L - 2), cur, kNH, kNH, rows, ns, ew, s->st));
         HIPCHK(reduce_partials(s->part.as<float>(), ns, kNH * kNH, G + offW(L), s->st));
@@ -269,8 +270,8 @@ atmo_status atmo_surrogate_create(atmo_surrogate **out, int device, uint64_t see
     HIPCHK(s->g.ensure(NPARAM * 4));
     HIPCHK(s->m.ensure(NPARAM * 4));
     HIPCHK(s->v.ensure(NPARAM * 4));
-    HIPCHK(s->wb.ensure((size_t)3 * kNH * kNH * 2));
-    HIPCHK(s->wt.ensure((size_t)3 * kNH * kNH * 2));
+    HIPCHK(s->wb.ensure((size_t)kNG * kNH * kNH * 2));
+    HIPCHK(s->wt.ensure((size_t)kNG * kNH * kNH * 2));
     HIPCHK(s->fz.ensure(FZ_BYTES));
     HIPCHK(s->loss.ensure(4));
     // Xavier-uniform weights, zero biases
@@ -282,7 +283,7 @@ atmo_status atmo_surrogate_create(atmo_surrogate **out, int device, uint64_t see
             h[off + i] = (float)(a * (2.0 * ((splitmix(rs) >> 11) * (1.0 / 9007199254740992.0)) - 1.0));
     };
     fill(OFF_W1, kNIN, kNH);
-    for (int L = 2; L <= 4; ++L) fill(offW(L), kNH, kNH);
+    for (int L = 2; L <= kNL; ++L) fill(offW(L), kNH, kNH);
     fill(OFF_W5, kNH, kNOUT);
     HIPCHK(hipMemcpy(s->p.p, h.data(), NPARAM * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(s->m.p, 0, NPARAM * 4));

@@ -9,7 +9,8 @@ namespace sg {
 constexpr int kNIN = 14;   // 4 texel coordinates + 7 atmosphere features + 3 exact-part values
 constexpr int kXS = 16;    // row stride of the input features (floats)
 constexpr int kNH = 256;   // hidden width
-constexpr int kNL = 4;     // hidden layers
+constexpr int kNL = 3;     // hidden layers (H1 .. H_kNL)
+constexpr int kNG = kNL - 1;   // hidden 256x256 layers (layers 2 .. kNL, on the matrix cores)
 constexpr int kNOUT = 4;   // outputs (rgb + an always-zero alpha)
 
 enum { EPI_FWD = 0, EPI_FWD_INF = 1, EPI_DX = 2, EPI_PARTIAL = 3 };
@@ -34,8 +35,8 @@ struct TargetNorm {
 // Weights of the fused inference kernel (device pointers), from prep_fused.
 struct FusedWeights {
     const __bf16 *a1h, *a1l;  // [256][16] layer 1 (column 14: bias), bf16 hi / lo
-    const __bf16 *wt;         // [3][256][256] layers 2..4 as [out][in]
-    const float *b234;        // [3][256] biases of layers 2..4
+    const __bf16 *wt;         // [kNG][256][256] layers 2..kNL as [out][in]
+    const float *bh;          // [kNG][256] biases of layers 2..kNL
     const __bf16 *a5h, *a5l;  // [4][256] layer 5 as [out][in], bf16 hi / lo
     const float *b5;          // [4]
 };
@@ -74,9 +75,9 @@ hipError_t code_rows(const FeatSet &fs, const unsigned *codes, int n, const floa
 // out[6] (device, zeroed by the caller): sums and sums of squares of the rgb targets
 hipError_t target_stats(const float *Tg, size_t n, double *out, hipStream_t st);
 hipError_t norm_targets(float *Tg, size_t n, const TargetNorm &tn, hipStream_t st);
-hipError_t prep_fused(const float *W1, const float *b1, const float *W5, const float *b2, const float *b3,
-                      const float *b4, __bf16 *a1h, __bf16 *a1l, __bf16 *a5h, __bf16 *a5l, float *b234,
-                      hipStream_t st);
+// bh0: bias of layer 2, the biases of layers 3..kNL follow at stride bstride floats
+hipError_t prep_fused(const float *W1, const float *b1, const float *W5, const float *bh0, size_t bstride,
+                      __bf16 *a1h, __bf16 *a1l, __bf16 *a5h, __bf16 *a5l, float *bh, hipStream_t st);
 // Fused inference of every S and E texel of a batch of sets (device array
 // `sets`, tile0 ascending, ntiles workgroups in all): adds the predicted
 // higher orders to S.rgb and E.

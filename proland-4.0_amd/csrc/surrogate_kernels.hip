@@ -1,7 +1,9 @@
 // surrogate_kernels.hip — kernels of the learned multiple-scattering surrogate.
 //
-// MLP 14 -> 256 -> 256 -> 256 -> 256 -> 4, SiLU on the hidden layers.  The
-// three 256x256 layers (97 % of the FLOPs) run on the BF16 matrix cores
+// MLP 14 -> 256 -> 256 -> 256 -> 4 (kNL = 3 hidden layers), SiLU on the hidden
+// layers.  Layers are numbered as for four hidden layers: the output layer is
+// "layer 5" and the last hidden layer's activations "H4" whatever kNL is.  The
+// kNG = kNL - 1 256x256 layers (96 % of the FLOPs) run on the BF16 matrix cores
 // (v_mfma_f32_32x32x16_bf16, FP32 accumulation) in one LDS-tiled GEMM kernel
 // with transposable operands and fused epilogues; the thin first and last
 // layers, the loss, the reductions and Adam are FP32 vector code.
@@ -655,7 +657,7 @@ template <int RBW, int NW>
 __global__ __launch_bounds__(64 * NW) void k_mlp_infer(const InferSet *__restrict__ sets, int nsets,
                                                        FusedWeights fw, TargetNorm tn) {
     extern __shared__ __attribute__((aligned(16))) __bf16 wl[];   // [256][kRowPad]
-    __shared__ float bs[3][kNH];
+    __shared__ float bs[kNG][kNH];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, h = lane >> 5, c = lane & 31;
     // the set of this workgroup: last set whose first tile <= blockIdx.x
     int lo = 0, hi = nsets - 1;
@@ -674,7 +676,7 @@ __global__ __launch_bounds__(64 * NW) void k_mlp_infer(const InferSet *__restric
     // layer-1 weights (hi then lo, [256][16]) into B, hidden biases; layer 2's first half -> A
     for (int i = threadIdx.x; i < 1024; i += 64 * NW)
         *(uint4 *)(wB + i * 8) = *(const uint4 *)((i < 512 ? fw.a1h : fw.a1l) + (i & 511) * 8);
-    for (int i = threadIdx.x; i < 3 * kNH; i += 64 * NW) (&bs[0][0])[i] = fw.b234[i];
+    for (int i = threadIdx.x; i < kNG * kNH; i += 64 * NW) (&bs[0][0])[i] = fw.bh[i];
     __syncthreads();
     stage_half_async<NW>(fw.wt, wl, 0, wv, lane);
 
@@ -713,16 +715,16 @@ __global__ __launch_bounds__(64 * NW) void k_mlp_infer(const InferSet *__restric
             tile_to_b(acc, nullptr, h, H[rb][2 * nt], H[rb][2 * nt + 1]);
         }
     }
-    // (2) hidden layers 2..4
+    // (2) hidden layers 2..kNL
 #pragma unroll 1
-    for (int L = 0; L < 3; ++L) {
+    for (int L = 0; L < kNG; ++L) {
         const __bf16 *W = fw.wt + (size_t)L * kNH * kNH;
         wait_all_and_sync();                          // A holds rows 0..127; B is free
         stage_half_async<NW>(W, wl, 128, wv, lane);   // B <- rows 128..255
         bf16x8 Hn[RBW][16];
         hidden_half<RBW>(wl, bs[L], 0, c, h, H, Hn);
         wait_all_and_sync();                          // B holds rows 128..255; A is free
-        if (L < 2) {
+        if (L < kNG - 1) {
             stage_half_async<NW>(W + (size_t)kNH * kNH, wl, 0, wv, lane);   // A <- next layer's rows 0..127
         } else if (threadIdx.x < 256) {               // A <- layer 5 (hi then lo, [4][256])
             *(uint4 *)(wl + threadIdx.x * 8) = *(const uint4 *)((threadIdx.x < 128 ? fw.a5h : fw.a5l) +
@@ -777,10 +779,9 @@ __global__ __launch_bounds__(64 * NW) void k_mlp_infer(const InferSet *__restric
 // Weights of the fused kernel from the FP32 parameters: layer 1 (+ bias in
 // column 14) and layer 5 split into bf16 hi/lo, the hidden biases packed.
 __global__ __launch_bounds__(256) void k_prep_fused(const float *__restrict__ W1, const float *__restrict__ b1,
-                                                    const float *__restrict__ W5, const float *__restrict__ b2,
-                                                    const float *__restrict__ b3, const float *__restrict__ b4,
-                                                    __bf16 *a1h, __bf16 *a1l, __bf16 *a5h, __bf16 *a5l,
-                                                    float *b234) {
+                                                    const float *__restrict__ W5, const float *__restrict__ bh0,
+                                                    size_t bstride, __bf16 *a1h, __bf16 *a1l, __bf16 *a5h,
+                                                    __bf16 *a5l, float *bh) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < kNH * 16) {
         const int nn = i / 16, k = i % 16;
@@ -794,9 +795,9 @@ __global__ __launch_bounds__(256) void k_prep_fused(const float *__restrict__ W1
         const __bf16 vh = (__bf16)v;
         a5h[j] = vh;
         a5l[j] = (__bf16)(v - (float)vh);
-    } else if (i < kNH * 16 + kNOUT * kNH + 3 * kNH) {
+    } else if (i < kNH * 16 + kNOUT * kNH + kNG * kNH) {
         const int j = i - kNH * 16 - kNOUT * kNH;
-        b234[j] = j < kNH ? b2[j] : (j < 2 * kNH ? b3[j - kNH] : b4[j - 2 * kNH]);
+        bh[j] = bh0[(size_t)(j / kNH) * bstride + j % kNH];
     }
 }
 
@@ -900,12 +901,11 @@ namespace sg {
 constexpr int kFusedRBW = 1, kFusedNW = 8;
 size_t fused_lds() { return (size_t)kNH * kRowPad * sizeof(__bf16); }
 int fused_rows_per_block() { return 32 * kFusedRBW * kFusedNW; }
-hipError_t prep_fused(const float *W1, const float *b1, const float *W5, const float *b2, const float *b3,
-                      const float *b4, __bf16 *a1h, __bf16 *a1l, __bf16 *a5h, __bf16 *a5l, float *b234,
-                      hipStream_t st) {
-    const int n = kNH * 16 + kNOUT * kNH + 3 * kNH;
-    hipLaunchKernelGGL(k_prep_fused, dim3(blocks(n)), dim3(256), 0, st, W1, b1, W5, b2, b3, b4, a1h, a1l, a5h,
-                       a5l, b234);
+hipError_t prep_fused(const float *W1, const float *b1, const float *W5, const float *bh0, size_t bstride,
+                      __bf16 *a1h, __bf16 *a1l, __bf16 *a5h, __bf16 *a5l, float *bh, hipStream_t st) {
+    const int n = kNH * 16 + kNOUT * kNH + kNG * kNH;
+    hipLaunchKernelGGL(k_prep_fused, dim3(blocks(n)), dim3(256), 0, st, W1, b1, W5, bh0, bstride, a1h, a1l, a5h,
+                       a5l, bh);
     return hipGetLastError();
 }
 hipError_t mlp_infer(const InferSet *sets, int nsets, int ntiles, const FusedWeights &fw, const TargetNorm &tn,

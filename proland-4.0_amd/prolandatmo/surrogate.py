@@ -8,6 +8,7 @@ from . import lib as L
 from .params import AtmoOptions, AtmoParams, options as default_options, params_array
 
 NIN, XS, NH, NOUT = 14, 16, 256, 4
+NL = 3   # hidden layers (H1 .. H_NL); layers 2 .. NL are 256x256
 
 
 class Report(ctypes.Structure):
@@ -70,9 +71,11 @@ def param_count():
 def layout():
     """name -> (offset, shape) of the FP32 parameter vector."""
     out, off = {}, 0
-    for name, shape in (("W1", (NIN, NH)), ("b1", (NH,)), ("W2", (NH, NH)), ("b2", (NH,)),
-                        ("W3", (NH, NH)), ("b3", (NH,)), ("W4", (NH, NH)), ("b4", (NH,)),
-                        ("W5", (NH, NOUT)), ("b5", (NOUT,))):
+    blocks = [("W1", (NIN, NH)), ("b1", (NH,))]
+    for L in range(2, NL + 1):   # the hidden 256x256 layers
+        blocks += [("W%d" % L, (NH, NH)), ("b%d" % L, (NH,))]
+    blocks += [("W5", (NH, NOUT)), ("b5", (NOUT,))]   # the output layer ("layer 5" for any NL)
+    for name, shape in blocks:
         out[name] = (off, shape)
         off += int(np.prod(shape))
     assert off == param_count()

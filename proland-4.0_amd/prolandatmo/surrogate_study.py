@@ -50,7 +50,7 @@ def run(a, log=_log):
         log("step %d loss %.4f (%.1f s)" % (k0 + n, parts[-1][-10:].mean(), time.perf_counter() - t))
     losses = np.concatenate(parts)
     dt = time.perf_counter() - t
-    flop = 3 * 2 * (SG.NIN * 256 + 3 * 256 * 256 + 256 * 4) * a.batch
+    flop = 3 * 2 * (SG.NIN * 256 + (SG.NL - 1) * 256 * 256 + 256 * 4) * a.batch
     res["training"] = {"steps": a.steps, "s": round(dt, 2), "ms_per_step": round(dt / a.steps * 1e3, 3),
                        "tflops_model": round(flop * a.steps / dt / 1e12, 1),
                        "loss_first": float(losses[:20].mean()), "loss_last": float(losses[-20:].mean()),

@@ -810,13 +810,15 @@ def test_inscatterN_forms_agree_on_the_whole_chain(eng):
         assert torch.equal(x, y) and torch.equal(y, z)
 
 
+@pytest.mark.parametrize("lin", [0, FLAG_LINEAR_INSCATTER])
 @pytest.mark.parametrize("res", [(4, 16, 32, 12), (4, 16, 48, 12), (4, 16, 64, 16)])
-def test_inscatterN_forms_agree_on_rows_of_more_than_256_texels(eng, res):
+def test_inscatterN_forms_agree_on_rows_of_more_than_256_texels(eng, res, lin):
     """Rows of 384, 576 and 1024 texels (one mu row per workgroup, 2, 3 and 4
     texels per thread), where the LDS-staged form blends all of a thread's
-    texels per sample at once: the same bits as the direct gathers."""
+    texels per sample at once: the same bits as the direct gathers, with the
+    non-linear and the linear inscatter parameterisation."""
     p = small(res=res)
-    o = dict(flags=8, orders=4)
+    o = dict(flags=8 | lin, orders=4)
     a = eng.precompute(p, options(inscatterN_form=1, **o))[0]
     b = eng.precompute(p, options(inscatterN_form=2, **o))[0]
     c = eng.precompute(p, options(inscatterN_form=3, **o))[0]

@@ -15,8 +15,8 @@ def params(seed=0, scale=0.2):
 
 def test_layout_size():
     lay, n = M.layout()
-    assert n == 202244
-    assert lay["W5"][0] == 14 * 256 + 256 + 3 * (256 * 256 + 256)
+    assert n == 14 * 256 + 256 + (M.NL - 1) * (256 * 256 + 256) + 256 * 4 + 4   # 136452 for NL = 3
+    assert lay["W5"][0] == 14 * 256 + 256 + (M.NL - 1) * (256 * 256 + 256)
 
 
 def test_backward_matches_finite_differences():
