@@ -10,9 +10,10 @@
  *
  * Network: 14 inputs (4 texel coordinates in [-1, 1], 7 normalised
  * atmosphere features, and the log-scaled rgb of the exactly computed orders
- * at the texel) -> 3 x 256 SiLU -> 4 outputs (log-scaled rgb of the
- * contribution of the orders above the exactly computed ones, standardised;
- * the 4th output is trained to 0 because those orders add nothing to alpha).
+ * at the texel) -> 3 x 256 (activation: z times a smoothstep gate of z) -> 4
+ * outputs (log-scaled rgb of the contribution of the orders above the exactly
+ * computed ones, standardised; the 4th output is trained to 0 because those
+ * orders add nothing to alpha).
  * The two 256x256 layers run in BF16 on the matrix cores with FP32
  * accumulation; master weights, Adam state, the first and last layers are
  * FP32.

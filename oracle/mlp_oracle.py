@@ -48,13 +48,15 @@ def bf16(x):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def silu(z):
-    return z / (1.0 + np.exp(-z))
+def act(z):
+    """Hidden activation: z times the smoothstep of t = clip(z / 6 + 1/2, 0, 1)."""
+    t = np.clip(z / 6.0 + 0.5, 0.0, 1.0)
+    return z * t * t * (3.0 - 2.0 * t)
 
 
-def dsilu(z):
-    s = 1.0 / (1.0 + np.exp(-z))
-    return s * (1.0 + z * (1.0 - s))
+def dact(z):
+    t = np.clip(z / 6.0 + 0.5, 0.0, 1.0)
+    return t * t * (3.0 - 2.0 * t) + z * t * (1.0 - t)
 
 
 def forward(p, X, emulate_bf16=True):
@@ -63,10 +65,10 @@ def forward(p, X, emulate_bf16=True):
     w = unpack(p)
     X = np.asarray(X, dtype=np.float64)[:, :NIN]
     Z = [X @ w["W1"] + w["b1"]]
-    H = [r(silu(Z[0]))]
+    H = [r(act(Z[0]))]
     for L in range(2, NL + 1):
         Z.append(H[-1] @ r(w["W%d" % L]) + w["b%d" % L])
-        H.append(r(silu(Z[-1])))
+        H.append(r(act(Z[-1])))
     Y = H[-1] @ w["W5"] + w["b5"]
     return Y, (X, Z, H, w)
 
@@ -80,12 +82,12 @@ def loss_grad(p, X, T, emulate_bf16=True):
     loss = float(np.sum(d * d) / (B * NOUT))
     dY = 2.0 * d / (B * NOUT)
     g = {"W5": H[NL - 1].T @ dY, "b5": dY.sum(0)}
-    dZ = r((dY @ w["W5"].T) * dsilu(Z[NL - 1]))
+    dZ = r((dY @ w["W5"].T) * dact(Z[NL - 1]))
     for L in range(NL, 1, -1):
         g["W%d" % L] = H[L - 2].T @ dZ
         g["b%d" % L] = dZ.sum(0)
         dH = dZ @ r(w["W%d" % L]).T
-        dZ = dH * dsilu(Z[L - 2])
+        dZ = dH * dact(Z[L - 2])
         if L > 2:
             dZ = r(dZ)
     g["W1"] = X.T @ dZ

@@ -1,6 +1,6 @@
 // surrogate_kernels.hip — kernels of the learned multiple-scattering surrogate.
 //
-// MLP 14 -> 256 -> 256 -> 256 -> 4 (kNL = 3 hidden layers), SiLU on the hidden
+// MLP 14 -> 256 -> 256 -> 256 -> 4 (kNL = 3 hidden layers), act() on the hidden
 // layers.  Layers are numbered as for four hidden layers: the output layer is
 // "layer 5" and the last hidden layer's activations "H4" whatever kNL is.  The
 // kNG = kNL - 1 256x256 layers (96 % of the FLOPs) run on the BF16 matrix cores
@@ -17,11 +17,22 @@ namespace sg {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
-__device__ __forceinline__ float sigm(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
-__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
-__device__ __forceinline__ float dsilu(float z) {
-    const float s = sigm(z);
-    return s * (1.0f + z * (1.0f - s));
+// Hidden activation: z gated by a smoothstep, act(z) = z s(t), t = clamp(z / 6
+// + 1/2, 0, 1), s = t^2 (3 - 2 t).  Like SiLU (z sigmoid(z)) it is C1 with
+// gate 1/2 and gate slope 1/4 at 0, but it is linear (z or 0) beyond |z| = 3 and
+// takes 6 full-rate instructions where SiLU needs two quarter-rate
+// transcendentals (exp, rcp) and three more: the activations of the fused
+// inference kernel are a bound of its speed.  act'(z) = s + z t (1 - t).
+__device__ __forceinline__ float gate_t(float z) {
+    return __builtin_amdgcn_fmed3f(fmaf(z, 1.0f / 6.0f, 0.5f), 0.0f, 1.0f);
+}
+__device__ __forceinline__ float act(float z) {
+    const float t = gate_t(z);
+    return z * (t * t * fmaf(-2.0f, t, 3.0f));
+}
+__device__ __forceinline__ float dact(float z) {
+    const float t = gate_t(z);
+    return fmaf(z, t * (1.0f - t), t * t * fmaf(-2.0f, t, 3.0f));
 }
 
 // ---------------------------------------------------------------------------
@@ -108,11 +119,11 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A,
                 if (EPI == EPI_FWD) {
                     const float z = v + ep.bias[col];
                     ep.zout[o] = z;
-                    ep.hout[o] = (__bf16)silu(z);
+                    ep.hout[o] = (__bf16)act(z);
                 } else if (EPI == EPI_FWD_INF) {
-                    ep.hout[o] = (__bf16)silu(v + ep.bias[col]);
+                    ep.hout[o] = (__bf16)act(v + ep.bias[col]);
                 } else if (EPI == EPI_DX) {
-                    const float g = v * dsilu(ep.zin[o]);
+                    const float g = v * dact(ep.zin[o]);
                     ep.hout[o] = (__bf16)g;
                     if (ep.zout) ep.zout[o] = g;
                 } else {  // EPI_PARTIAL
@@ -122,7 +133,7 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A,
 }
 
 // ---------------------------------------------------------------------------
-// Layer 1 (14 -> 256), FP32: Z1 = X W1 + b1, H1 = SiLU(Z1).  X rows are 16
+// Layer 1 (14 -> 256), FP32: Z1 = X W1 + b1, H1 = act(Z1).  X rows are 16
 // floats (14 used).  One thread per (row, column).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_layer1_fwd(const float *__restrict__ X,
@@ -137,7 +148,7 @@ __global__ __launch_bounds__(256) void k_layer1_fwd(const float *__restrict__ X,
 #pragma unroll
     for (int i = 0; i < kNIN; ++i) z = fmaf(X[b * kXS + i], W1[i * kNH + n], z);
     if (Z1) Z1[t] = z;
-    H1[t] = (__bf16)silu(z);
+    H1[t] = (__bf16)act(z);
 }
 
 // ---------------------------------------------------------------------------
@@ -194,7 +205,7 @@ __global__ __launch_bounds__(256) void k_layer5(const __bf16 *__restrict__ H4,
     if (threadIdx.x == 0) atomicAdd(loss, (lsum[0] + lsum[1] + lsum[2] + lsum[3]) * inv);
 }
 
-// Layer 5 backward: dW5 += H4^T dY, db5 += sum dY, dZ4 = (dY W5^T) * SiLU'(Z4).
+// Layer 5 backward: dW5 += H4^T dY, db5 += sum dY, dZ4 = (dY W5^T) * act'(Z4).
 // A wave per row at a time as in the forward.
 __global__ __launch_bounds__(256) void k_layer5_bwd(const __bf16 *__restrict__ H4,
                                                     const float *__restrict__ Z4,
@@ -226,7 +237,7 @@ __global__ __launch_bounds__(256) void k_layer5_bwd(const __bf16 *__restrict__ H
                 part[wv][k * kNOUT + o] += h * d[o];
                 g = fmaf(d[o], W5[k * kNOUT + o], g);
             }
-            dZ4[idx] = (__bf16)(g * dsilu(Z4[idx]));
+            dZ4[idx] = (__bf16)(g * dact(Z4[idx]));
         }
     }
     if (lane < kNOUT) atomicAdd(&gb5[lane], dsum);
@@ -526,7 +537,7 @@ __global__ __launch_bounds__(256) void k_absmax(const float *__restrict__ a, siz
 }
 
 // ---------------------------------------------------------------------------
-// Fused inference: texel features -> layer 1 -> 3 x (256x256 + bias + SiLU)
+// Fused inference: texel features -> layer 1 -> kNG x (256x256 + bias + act)
 // -> layer 5 -> orders >= 2 added to S / written to E, all in one kernel with
 // the activations in registers.
 //
@@ -554,7 +565,7 @@ __device__ __forceinline__ unsigned pack_bf16(float a, float b) {
     return lo | (hi << 16);
 }
 
-// D tile (features x 32 rows) -> SiLU(D + bias) as the B operands of k blocks
+// D tile (features x 32 rows) -> act(D + bias) as the B operands of k blocks
 // 2nt (lo) and 2nt + 1 (hi).  bias: the tile's 32 biases or nullptr.
 __device__ __forceinline__ void tile_to_b(const f32x16 &acc, const float *bias, int h, bf16x8 &lo,
                                           bf16x8 &hi) {
@@ -565,7 +576,7 @@ __device__ __forceinline__ void tile_to_b(const f32x16 &acc, const float *bias, 
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const float z = acc[4 * q + t] + (bias ? bias[8 * q + 4 * h + t] : 0.0f);
-            a[t] = silu(z);
+            a[t] = act(z);
         }
         P[q][0] = pack_bf16(a[0], a[1]);
         P[q][1] = pack_bf16(a[2], a[3]);
@@ -610,7 +621,7 @@ __device__ __forceinline__ void stage_half_async(const __bf16 *W, __bf16 *wl, in
 
 // Output tiles nt0 .. nt0+3 of one hidden layer from the staged weight rows,
 // software-pipelined: the matrix-core products of tile nt run while the
-// vector units apply bias + SiLU and relayout tile nt - 1.
+// vector units apply bias + act and relayout tile nt - 1.
 template <int RBW>
 __device__ __forceinline__ void tile_mfma(const __bf16 *wl, int nt, int c, int h, const bf16x8 (&H)[RBW][16],
                                           f32x16 (&acc)[RBW]) {
