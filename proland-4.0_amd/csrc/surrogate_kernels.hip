@@ -403,7 +403,8 @@ __device__ __forceinline__ unsigned hash32(unsigned long long z) {
     return (unsigned)((z ^ (z >> 31)) >> 32);
 }
 
-// Training rows of one set: `ns` seeded S texels then all E texels.
+// Training rows of one set: `ns` seeded S texels (a quarter of them near the
+// horizon) then all E texels.
 // target c = (log(max(v_c, 0) + eps) - mu_c) / sd_c, v = S(orders<=N) - S(orders<=K) for S
 // texels (rgb) and E(orders<=N) - E(orders<=K) for E texels, K the exactly computed
 // orders (S1, E1); the fourth target is 0 (the orders above 1 add 0 to alpha).
@@ -419,7 +420,17 @@ __global__ __launch_bounds__(256) void k_make_rows(FeatSet fs, const float4 *__r
     unsigned code;
     float v[3], ex[3];
     if (t < ns) {
-        code = hash32(seed * 0x100000001ull + t) % (unsigned)fs.n4;
+        const unsigned hh = hash32(seed * 0x100000001ull + t);
+        if ((t & 3) == 0 && fs.RMU >= 8) {
+            // a quarter of the samples from the 8 mu rows around the horizon,
+            // where each set's worst texels sit (the last ground rows, the
+            // first sky rows); the rest uniform over the table
+            const unsigned W4 = (unsigned)(fs.RMUS * fs.RNU);
+            const unsigned l = hh % (unsigned)fs.RR, y = (unsigned)(fs.RMU / 2 - 4) + (hh / (unsigned)fs.RR) % 8u;
+            code = (l * (unsigned)fs.RMU + y) * W4 + (hh / ((unsigned)fs.RR * 8u)) % W4;
+        } else {
+            code = hh % (unsigned)fs.n4;
+        }
         const float4 a = S4[code], b = S1[code];
         v[0] = a.x - b.x; v[1] = a.y - b.y; v[2] = a.z - b.z;
         ex[0] = b.x; ex[1] = b.y; ex[2] = b.z;
