@@ -34,6 +34,8 @@ def run(a, log=_log):
     o = options()
     s = SG.Surrogate(0, seed=0x5EED0004)
     s.set_exact_orders(a.exact_orders)
+    if a.eps is not None:   # the floor of the log-scaled targets (before standardisation)
+        s.set_norm([0.0, 0.0, 0.0], [1.0, 1.0, 1.0], a.eps)
     train_sets, _ = draw_sets(SEED_TRAIN, a.train_sets)
     t = time.perf_counter()
     rows = s.build_dataset(train_sets, o, samples_per_set=a.samples, seed=SEED_TRAIN)
@@ -145,6 +147,8 @@ def parser():
     ap.add_argument("--exact-orders", type=int, default=1,
                     help="orders the plain passes compute (1 or 2); the network predicts the rest")
     ap.add_argument("--tol", type=float, default=1e-3)
+    ap.add_argument("--eps", type=float, default=None,
+                    help="floor of the log-scaled training targets (default: the library's, 1e-6)")
     ap.add_argument("--out", default=None)
     ap.add_argument("--save-params", default=None)
     return ap
