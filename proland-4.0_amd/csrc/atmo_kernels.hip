@@ -294,9 +294,11 @@ __global__ __launch_bounds__(256) void k_inscatter1(const SetDesc *__restrict__ 
 // separable, so the 4D lookup of every direction (theta, phi) reduces to a
 // 2-entry lerp of LDS data once those slice values are staged.
 //
-// Staged pair layout: P[it][kk] = {A = V[kk-1], D = V[kk] - V[kk-1]} for
-// kk = 0..RNU, so lookup(nu) = A + f D with pos = (nu+1)/2 (RNU-1) + 1,
-// kk = floor(pos), f = pos - kk.
+// Staged pairs per theta row, kk = 0..RNU, with pos = (nu+1)/2 (RNU-1) + 1,
+// kk = floor(pos), f = pos - kk, A = V[kk-1], D = V[kk] - V[kk-1]:
+// irradianceN stores {A, D} and looks up A + f D; inscatterS stores
+// {A - kk D, D} (rounded once from FP64) and looks up fma(pos, D, A - kk D),
+// the same line without the fraction.
 // ---------------------------------------------------------------------------
 template <int NT>
 __device__ __forceinline__ void stage_slice(const SetDesc &s, const float4 *const *tabs,
@@ -343,6 +345,12 @@ __host__ __device__ __forceinline__ int e_pad(int SW) { return (2 * SW) / 3 + 3;
 // [1, RNU] up to FP32 rounding (|nu1| <= 1 + a few ulp); the merged ground rows
 // cover [kPosLo, RNU + kPosHi].
 constexpr float kPosLo = 1.0f - 1.0f / 256.0f, kPosHi = 1.0f / 256.0f;
+
+// The value at 0 of the line through (k, a) with slope d, a - k d, from FP64
+// and rounded once; a lookup at pos in [k, k + 1) is then fma(pos, d, .).
+__device__ __forceinline__ float at_zero(float a, float d, int k) {
+    return (float)((double)a - (double)k * (double)d);
+}
 
 struct DirJ {      // per-theta constants of the NS x 2NS grid
     float ct, st, dw, pad;
@@ -396,8 +404,10 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
         const int k0 = min(max(k - EPAD, 0), s.SW - 1), k1 = min(max(k - EPAD + 1, 0), s.SW - 1);
         const float4 a = s.dE4[k0];
         const float4 b = s.dE4[k1];
-        epair[2 * k] = make_float4(a.x, a.y, a.z, b.x - a.x);
-        epair[2 * k + 1] = make_float4(b.y - a.y, b.z - a.z, 0.f, 0.f);
+        // {a - k d, d}: the lookup at texel coordinate xg is fma(xg, d, a - k d)
+        const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+        epair[2 * k] = make_float4(at_zero(a.x, dx, k), at_zero(a.y, dy, k), at_zero(a.z, dz, k), dx);
+        epair[2 * k + 1] = make_float4(dy, dz, 0.f, 0.f);
     }
 
     // per-layer scattering coefficients (phase constants folded in)
@@ -506,26 +516,29 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
             const float4 a = vtmp[((size_t)it * NV + kk) * NT];
             const float4 b = vtmp[((size_t)it * NV + kk + 1) * NT];
             float4 *q = pairs + ((size_t)rowoff(it) + kk) * PW;
+            const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+            const float px = at_zero(a.x, dx, kk), py = at_zero(a.y, dy, kk), pz = at_zero(a.z, dz, kk);
             if (!FIRST) {
-                // (V.rgb, D.r), (D.g, D.b): one 16-byte and one 8-byte LDS read
-                q[0] = make_float4(a.x, a.y, a.z, b.x - a.x);
-                q[1] = make_float4(b.y - a.y, b.z - a.z, 0.f, 0.f);
+                // (P.rgb, D.r), (D.g, D.b): one 16-byte and one 8-byte LDS read
+                q[0] = make_float4(px, py, pz, dx);
+                q[1] = make_float4(dy, dz, 0.f, 0.f);
             } else {
                 // Rayleigh and Mie pairs in 48 bytes: three 16-byte LDS reads
                 const float4 am = vtmp[((size_t)it * NV + kk) * NT + 1];
                 const float4 bm = vtmp[((size_t)it * NV + kk + 1) * NT + 1];
-                q[0] = make_float4(a.x, a.y, a.z, b.x - a.x);
-                q[1] = make_float4(b.y - a.y, b.z - a.z, am.x, am.y);
-                q[2] = make_float4(am.z, bm.x - am.x, bm.y - am.y, bm.z - am.z);
+                const float ex = bm.x - am.x, ey = bm.y - am.y, ez = bm.z - am.z;
+                q[0] = make_float4(px, py, pz, dx);
+                q[1] = make_float4(dy, dz, at_zero(am.x, ex, kk), at_zero(am.y, ey, kk));
+                q[2] = make_float4(at_zero(am.z, ez, kk), ex, ey, ez);
             }
         }
         // (4b) orders >= 3, ground rows: the lookup of a ground direction is
         // S(pos) + g0 E(xg(pos)) with xg = c1 pos + ce0, two piecewise-linear
         // functions of the same position, with knots at the integers (slices) and
         // where xg crosses an integer (E texels).  The merged row holds one entry
-        // per segment between consecutive knots of either kind: the value P at
-        // the segment's start p0 and the slope Q, so that a lookup is
-        // P + (pos - p0) Q.  The segment of (kk, jg) = (floor(pos), floor(xg)) is
+        // per segment between consecutive knots of either kind: the slope Q and
+        // P - p0 Q, where P is the value at the segment's start p0, so that a
+        // lookup is fma(pos, Q, P - p0 Q).  The segment of (kk, jg) = (floor(pos), floor(xg)) is
         // entry kk + jg - jlo (each knot raises one of the two by one); segments
         // are found by intersecting every slice cell with every E cell of the row
         // (FP64), so entries of zero-length segments at coinciding knots exist too.
@@ -547,7 +560,6 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
                     hi = fmin(hi, (jg + 1 - ce0) / c1);
                 }
                 if (lo > hi) continue;
-                lo = (double)(float)lo;   // P is the value at the stored (FP32) start
                 const float4 a = vtmp[((size_t)itl * NV + kk) * NT];
                 const float4 b = vtmp[((size_t)itl * NV + kk + 1) * NT];
                 const float4 ea = s.dE4[min(max(jg - EPAD, 0), s.SW - 1)];
@@ -558,10 +570,12 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
                 const double Px = a.x + u * dx + g0.x * (ea.x + v * fx);
                 const double Py = a.y + u * dy + g0.y * (ea.y + v * fy);
                 const double Pz = a.z + u * dz + g0.z * (ea.z + v * fz);
-                const double Qx = dx + g0.x * c1 * fx, Qy = dy + g0.y * c1 * fy, Qz = dz + g0.z * c1 * fz;
+                const float Qx = (float)(dx + g0.x * c1 * fx), Qy = (float)(dy + g0.y * c1 * fy),
+                            Qz = (float)(dz + g0.z * c1 * fz);
+                // the FP32 slope's line through (p0, P), at pos = 0
                 float4 *q = pairs + ((size_t)rowoff(itl) + kk + jj) * PW;
-                q[0] = make_float4((float)Px, (float)Py, (float)Pz, (float)Qx);
-                q[1] = make_float4((float)Qy, (float)Qz, (float)lo, 0.f);
+                q[0] = make_float4((float)(Px - lo * Qx), (float)(Py - lo * Qy), (float)(Pz - lo * Qz), Qx);
+                q[1] = make_float4(Qy, Qz, 0.f, 0.f);
             }
         }
         __syncthreads();
@@ -618,14 +632,15 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
             // the caller applies q1, whose per-set constants are folded into the staging)
             auto lookup = [&](const float4 *pt, float pos, float out[3], float &q1) {
                 const unsigned kk = __float2uint_rz(pos);
-                const float f = __builtin_amdgcn_fractf(pos);
-                const float4 *pp = pt + kk * PW;
+                // PW = 3: a 24-bit multiply (full rate; kk <= RNU) instead of the
+                // 32-bit one the compiler picks; PW = 2: a shift
+                const float4 *pp = pt + (FIRST ? __umul24(kk, (unsigned)PW) : kk * PW);
                 const float4 a = pp[0];
                 if (!FIRST) {
                     const float2 d = *(const float2 *)(pp + 1);
-                    out[0] = fmaf(f, a.w, a.x);
-                    out[1] = fmaf(f, d.x, a.y);
-                    out[2] = fmaf(f, d.y, a.z);
+                    out[0] = fmaf(pos, a.w, a.x);
+                    out[1] = fmaf(pos, d.x, a.y);
+                    out[2] = fmaf(pos, d.y, a.z);
                     q1 = 1.0f;
                 } else {
                     const float4 b = pp[1], c2 = pp[2];
@@ -633,10 +648,10 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
                     q1 = fmaf(nu1, nu1, 1.0f);
                     const float y1 = __builtin_amdgcn_rsqf(fmaf(m2g, nu1, g2p1));
                     const float z1 = y1 * y1 * y1;
-                    const float m0 = fmaf(f, c2.y, b.z), m1 = fmaf(f, c2.z, b.w), m2 = fmaf(f, c2.w, c2.x);
-                    out[0] = fmaf(m0, z1, fmaf(f, a.w, a.x));
-                    out[1] = fmaf(m1, z1, fmaf(f, b.x, a.y));
-                    out[2] = fmaf(m2, z1, fmaf(f, b.y, a.z));
+                    const float m0 = fmaf(pos, c2.y, b.z), m1 = fmaf(pos, c2.z, b.w), m2 = fmaf(pos, c2.w, c2.x);
+                    out[0] = fmaf(m0, z1, fmaf(pos, a.w, a.x));
+                    out[1] = fmaf(m1, z1, fmaf(pos, b.x, a.y));
+                    out[2] = fmaf(m2, z1, fmaf(pos, b.y, a.z));
                 }
             };
             // orders >= 3, ground direction: the merged sky + ground entry of the
@@ -645,22 +660,21 @@ __global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ 
                 const float xg = fmaf(pos, cg1, cg0);   // >= 0: EPAD is folded into cg0
                 const unsigned e = __float2uint_rz(pos) + __float2uint_rz(xg);
                 const float4 *pp = pairs + (gb + e * PW);
-                const float4 a = pp[0], b = pp[1];
-                const float f = pos - b.z;
-                out[0] = fmaf(f, a.w, a.x);
-                out[1] = fmaf(f, b.x, a.y);
-                out[2] = fmaf(f, b.y, a.z);
+                const float4 a = pp[0];
+                const float2 b = *(const float2 *)(pp + 1);
+                out[0] = fmaf(pos, a.w, a.x);
+                out[1] = fmaf(pos, b.x, a.y);
+                out[2] = fmaf(pos, b.y, a.z);
             };
             // irradiance at the ground point seen in direction w, row 0 of dE
             auto ground_e = [&](float pos, float cg1, float cg0, float out[3]) {
                 const float xg = fmaf(pos, cg1, cg0);   // >= 0: EPAD is folded into cg0
                 const unsigned kg = __float2uint_rz(xg);
-                const float fg = __builtin_amdgcn_fractf(xg);
                 const float4 ea = epair[2 * kg];
                 const float2 ed = *(const float2 *)(epair + 2 * kg + 1);
-                out[0] = fmaf(fg, ea.w, ea.x);
-                out[1] = fmaf(fg, ed.x, ea.y);
-                out[2] = fmaf(fg, ed.y, ea.z);
+                out[0] = fmaf(xg, ea.w, ea.x);
+                out[1] = fmaf(xg, ed.x, ea.y);
+                out[2] = fmaf(xg, ed.y, ea.z);
             };
 
             for (int itl = 0; itl < nth; ++itl) {
