@@ -282,12 +282,12 @@ struct Region {
 // k_inscatterS texels per thread for one set: a workgroup integrates the
 // RES_MU x RES_NU texels of one (r, mu_s) column, in RES_MU * ceil(RES_NU/G)
 // groups of G texels over 256 threads, so its time is about
-// ceil(groups / 256) * G; the smallest such G in {4, 3, 2} (4 on ties, whose
-// per-direction terms are shared most widely).
+// ceil(groups / 256) * G; the smallest such G in {8, 4, 3, 2} (the largest on
+// ties, whose per-direction terms are shared most widely).
 int jg_for_set(const atmo_params &p) {
-    int best = 4;
+    int best = 8;
     long best_t = 1L << 40;
-    for (int g : {4, 3, 2}) {
+    for (int g : {8, 4, 3, 2}) {
         const long groups = (long)p.RES_MU * ((p.RES_NU + g - 1) / g);
         const long t = ((groups + 255) / 256) * g;
         if (t < best_t) {

@@ -361,8 +361,12 @@ struct DirJ {      // per-theta constants of the NS x 2NS grid
 // in the dJ table itself and resumes them, so the sum over theta runs in the
 // same order, and gives the same bits, as with all rows staged at once
 // (!CHUNK, CH = NS: one round, compiled without the chunk bookkeeping).
+// At most 128 VGPRs (four waves per SIMD, as the LDS of a typical staging
+// allows): G = 8 then keeps its direction loops free of spills (its per-group
+// set-up spills a few registers) and runs 2 % faster than G = 4 at 32 x 64
+// directions; at 166 VGPRs (three waves) it ran 14 % slower.
 template <bool FIRST, int G, bool CHUNK>
-__global__ __launch_bounds__(256) void k_inscatterS(const SetDesc *__restrict__ sets,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_inscatterS(const SetDesc *__restrict__ sets,
                                                     const uint2 *__restrict__ tiles, int NS, int CH,
                                                     int JX, const DirJ *__restrict__ dirT,
                                                     const float2 *__restrict__ dirW) {
