@@ -54,6 +54,7 @@ atmo_status fail(atmo_status st, const char *fmt, ...) {
 enum Pass { P_T = 0, P_E1, P_S1, P_J, P_EN, P_SN, P_CHECK, P_COUNT };
 
 constexpr size_t kXcds = 8;  // MI355X accelerator complex dies
+constexpr size_t kJBlock = 16;   // dJ tiles (mu_s columns) dealt to one XCD together
 
 atmo_options default_options() {
     atmo_options o;
@@ -543,6 +544,27 @@ void build_tiles(const std::vector<atmo_params> &ps, const atmo_options &o,
                 if (jseg->empty() || jseg->back().G != g) jseg->push_back({g, (int)i, 0});
                 jseg->back().cnt += 1;
             }
+        }
+        if (pass == P_J) {
+            // Consecutive tiles are neighbouring mu_s columns of one layer, whose
+            // 12-byte dJ texels share cache lines.  Deal them to the XCDs in blocks
+            // of kJBlock, so that a block runs on one XCD (one L2) at about the same
+            // time and the partial lines merge there before they are written back;
+            // block-cyclic, so the order (per-set G groups, expensive sets first)
+            // holds at block granularity.  Per launch segment (each jseg is its own
+            // launch, whose workgroup ids start at 0).
+            auto deal = [&](size_t b0, size_t n) {
+                std::vector<std::vector<std::pair<double, uint2>>> q(kXcds);
+                for (size_t i = 0; i < n; ++i) q[(i / kJBlock) % kXcds].push_back(v[b0 + i]);
+                size_t p = b0;
+                for (size_t k = 0; p < b0 + n; ++k)
+                    for (size_t x = 0; x < kXcds; ++x)
+                        if (k < q[x].size()) v[p++] = q[x][k];
+            };
+            if (jseg && !jseg->empty())
+                for (const JSeg &g : *jseg) deal((size_t)g.off, (size_t)g.cnt);
+            else
+                deal(0, v.size());
         }
         for (auto &e : v) tiles.push_back(e.second);
         cnt[pass] = (int)tiles.size() - off[pass];
