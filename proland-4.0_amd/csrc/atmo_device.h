@@ -120,6 +120,20 @@ struct F4 {
 typedef __attribute__((address_space(1))) const F4 gfloat4;
 __device__ __forceinline__ const gfloat4 *gptr(const float4 *p) { return (const gfloat4 *)p; }
 __device__ __forceinline__ float3 ld3(const gfloat4 *p) { return make_float3(p->x, p->y, p->z); }
+
+// Raw buffer loads of a table: the descriptor (base address, size in bytes) sits
+// in scalar registers, and a load's byte offset is one VGPR plus one SGPR, so a
+// gather needs no 64-bit address arithmetic per load.  The fourth descriptor word
+// is the gfx9 one for untyped access (data format 32).  A load past `bytes`
+// returns zeros, which no caller relies on.
+typedef float buf_f3 __attribute__((ext_vector_type(3)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t table_rsrc(const void *p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float3 ld3b(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    const buf_f3 v = __builtin_amdgcn_raw_buffer_load_b96(r, voff, soff, 0);
+    return make_float3(v.x, v.y, v.z);
+}
 __device__ __forceinline__ float3 lerp3(float3 a, float3 b, float t) {
     return make_float3(fmaf(t, b.x - a.x, a.x), fmaf(t, b.y - a.y, a.y), fmaf(t, b.z - a.z, a.z));
 }

@@ -1050,8 +1050,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         }
     }
 
-    typedef __attribute__((address_space(1))) const char gchar;
-    gchar *J = (gchar *)s.dJ4;
+    // the dJ table as a buffer (< 2^31 bytes: at most 2^27 texels of 16 bytes)
+    const __amdgpu_buffer_rsrc_t Jr = table_rsrc(s.dJ4, (unsigned)((size_t)s.RR * HW * s.dJb));
     // A lookup's texel column is kbase + txs: kbase the first column of its nu
     // slice, txs its offset, kept apart so that the fraction ax has full FP32
     // precision.  txs runs past RMU_S - 1 into the next slice when mu_s_i > 1
@@ -1128,10 +1128,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     // the blended columns i0 and i0 + 1 (as the staged form's rows hold them)
                     auto column = [&](unsigned x) {
                         const unsigned o = x * jb;
-                        return sn_blend(en.w, ld3((const gfloat4 *)(J + (unsigned)en.base.x + o)),
-                                        ld3((const gfloat4 *)(J + (unsigned)en.base.y + o)),
-                                        ld3((const gfloat4 *)(J + (unsigned)en.base.z + o)),
-                                        ld3((const gfloat4 *)(J + (unsigned)en.base.w + o)));
+                        return sn_blend(en.w, ld3b(Jr, (unsigned)en.base.x + o, 0u), ld3b(Jr, (unsigned)en.base.y + o, 0u),
+                                        ld3b(Jr, (unsigned)en.base.z + o, 0u), ld3b(Jr, (unsigned)en.base.w + o, 0u));
                     };
                     float3 v = sn_lerp(ax, column(i0), column(i0 + 1));
                     if (fk[j] != 0.0f) {  // off-grid nu: blend with the next slice (k0 + 1 <= RNU - 1)
@@ -1159,15 +1157,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     for (int k = 0; k < ns; ++k) {
                         const float4 w = ent[0][s0 + k].w;
                         const int4 base = ent[0][s0 + k].base;
+                        // the 4 row starts are the same for the whole workgroup: in
+                        // scalar registers, the loads take them as their base and
+                        // the thread's own column offset (fixed over the samples)
+                        const unsigned R0 = (unsigned)__builtin_amdgcn_readfirstlane(base.x);
+                        const unsigned R1 = (unsigned)__builtin_amdgcn_readfirstlane(base.y);
+                        const unsigned R2 = (unsigned)__builtin_amdgcn_readfirstlane(base.z);
+                        const unsigned R3 = (unsigned)__builtin_amdgcn_readfirstlane(base.w);
                         float3 t[NM][4];
 #pragma unroll
                         for (int j = 0; j < NM; ++j) {
                             // past the row's end: a valid address, the blend is not stored
                             const unsigned xo = (unsigned)min((int)threadIdx.x + j * (int)blockDim.x, per - 1) * jb;
-                            t[j][0] = ld3((const gfloat4 *)(J + (unsigned)base.x + xo));
-                            t[j][1] = ld3((const gfloat4 *)(J + (unsigned)base.y + xo));
-                            t[j][2] = ld3((const gfloat4 *)(J + (unsigned)base.z + xo));
-                            t[j][3] = ld3((const gfloat4 *)(J + (unsigned)base.w + xo));
+                            t[j][0] = ld3b(Jr, xo, R0);
+                            t[j][1] = ld3b(Jr, xo, R1);
+                            t[j][2] = ld3b(Jr, xo, R2);
+                            t[j][3] = ld3b(Jr, xo, R3);
                         }
 #pragma unroll
                         for (int j = 0; j < NM; ++j) {
@@ -1232,11 +1237,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 float3 t[P][4];
 #pragma unroll
                 for (int q = 0; q < P; ++q) {
+                    // the row starts differ between the rows of a workgroup: in the VGPR offset
                     const int4 base = er[s0 + k + q].base;
-                    t[q][0] = ld3((const gfloat4 *)(J + (unsigned)base.x + xo));
-                    t[q][1] = ld3((const gfloat4 *)(J + (unsigned)base.y + xo));
-                    t[q][2] = ld3((const gfloat4 *)(J + (unsigned)base.z + xo));
-                    t[q][3] = ld3((const gfloat4 *)(J + (unsigned)base.w + xo));
+                    t[q][0] = ld3b(Jr, (unsigned)base.x + xo, 0u);
+                    t[q][1] = ld3b(Jr, (unsigned)base.y + xo, 0u);
+                    t[q][2] = ld3b(Jr, (unsigned)base.z + xo, 0u);
+                    t[q][3] = ld3b(Jr, (unsigned)base.w + xo, 0u);
                 }
 #pragma unroll
                 for (int q = 0; q < P; ++q) {
