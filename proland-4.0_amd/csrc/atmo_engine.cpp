@@ -334,6 +334,7 @@ struct Plan {
     size_t dirJT_off = 0, dirJW_off = 0, dirE_off = 0;
     size_t ldsJ1 = 0, ldsJ = 0, ldsE1 = 0, ldsEN = 0;
     int chJ1 = 0, chJ = 0, chE1 = 0, chEN = 0;   // theta rows per LDS staging round
+    int maxNU = 0;     // largest RES_NU of the batch (the dJ staging's entry count)
     int jx = 0;   // extra LDS entries of the merged ground rows of an orders >= 3 dJ round
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -579,10 +580,10 @@ hipError_t launch_dj(const Plan &pl, const SetDesc *S, const uint2 *T, const cha
     const size_t lds = first ? pl.ldsJ1 : pl.ldsJ;
     const int ch = first ? pl.chJ1 : pl.chJ;
     if (G != 0)
-        return launch_inscatterS(S, T + pl.off[P_J], pl.cnt[P_J], o.spherical_samples, ch, pl.jx,
+        return launch_inscatterS(S, T + pl.off[P_J], pl.cnt[P_J], o.spherical_samples, ch, pl.maxNU, pl.jx,
                                  dirs + pl.dirJT_off, dirs + pl.dirJW_off, first, lds, G, st);
     for (const JSeg &g : pl.jseg) {
-        hipError_t e = launch_inscatterS(S, T + pl.off[P_J] + g.off, g.cnt, o.spherical_samples, ch, pl.jx,
+        hipError_t e = launch_inscatterS(S, T + pl.off[P_J] + g.off, g.cnt, o.spherical_samples, ch, pl.maxNU, pl.jx,
                                          dirs + pl.dirJT_off, dirs + pl.dirJW_off, first, lds, g.G, st);
         if (e != hipSuccess) return e;
     }
@@ -889,8 +890,9 @@ atmo_status build_plan(hipStream_t xs, const atmo_params *sets, int nsets, const
     pl->chEN = chunk_irradianceN(o.irradiance_samples, maxNU, false);
     if (!pl->chJ1 || !pl->chJ || !pl->chE1 || !pl->chEN)
         return fail(ATMO_ERR_UNSUPPORTED, "SKY_W %d: the ground irradiance row does not fit in LDS", maxSW);
-    pl->ldsJ1 = lds_inscatterS(pl->chJ1, maxNU, maxSW, 0, true);
-    pl->ldsJ = lds_inscatterS(pl->chJ, maxNU, maxSW, pl->jx, false);
+    pl->maxNU = maxNU;
+    pl->ldsJ1 = lds_inscatterS(o.spherical_samples, pl->chJ1, maxNU, maxSW, 0, true);
+    pl->ldsJ = lds_inscatterS(o.spherical_samples, pl->chJ, maxNU, maxSW, pl->jx, false);
     pl->ldsE1 = lds_irradianceN(pl->chE1, maxNU, true);
     pl->ldsEN = lds_irradianceN(pl->chEN, maxNU, false);
     out = std::move(pl);
@@ -1335,8 +1337,8 @@ static atmo_status pass_run(atmo_engine *e, const atmo_params *p, const atmo_opt
         const int ch = chunk_inscatterS(o.spherical_samples, p->RES_NU, p->SKY_W, jx_of_ch, first);
         if (!ch) return fail(ATMO_ERR_UNSUPPORTED, "SKY_W %d: the ground irradiance row does not fit in LDS", p->SKY_W);
         const int jx = first ? 0 : jx_of_ch(ch);
-        const size_t lds = lds_inscatterS(ch, p->RES_NU, p->SKY_W, jx, first);
-        HIPCHK(launch_inscatterS(S, T + off[P_J], cnt[P_J], o.spherical_samples, ch, jx, d + oJT, d + oJW, first,
+        const size_t lds = lds_inscatterS(o.spherical_samples, ch, p->RES_NU, p->SKY_W, jx, first);
+        HIPCHK(launch_inscatterS(S, T + off[P_J], cnt[P_J], o.spherical_samples, ch, p->RES_NU, jx, d + oJT, d + oJW, first,
                                  lds, o.texels_per_thread ? o.texels_per_thread : jg_for_set(*p), hs));
         break;
     }

@@ -26,7 +26,14 @@ constexpr size_t kSphLdsBudget = 64 * 1024;
 // JX: extra LDS entries of the merged ground rows of one staging round of the
 // orders >= 3 pass (the sum over its theta rows of merged_row_extra, maximum
 // over the layers and sets of a launch)
-size_t lds_inscatterS(int CH, int RNU, int SW, int JX, bool first);
+// Order 2 stages its 48-byte entries as three planes of float4, kJ1Plane
+// entries apart, in unchunked launches whose staging round (all NS rows) fits
+// one plane; other launches keep each entry whole.  544 = 32 x 64 directions at
+// RES_NU 16.  Chunked: CH < NS, or an order-2 round too large for a plane
+// (those launches use the chunked variant, also for a single round).
+constexpr int kJ1Plane = 544;
+bool inscatterS_chunked(int NS, int CH, int RNU, bool first);
+size_t lds_inscatterS(int NS, int CH, int RNU, int SW, int JX, bool first);
 // bound on the extra entries of theta row `it` (NS rows) of a layer at radius r
 int merged_row_extra(double r, double Rg, double Rt, int SW, int NS, int it);
 size_t lds_irradianceN(int CH, int RNU, bool first);
@@ -43,9 +50,10 @@ hipError_t launch_tground(const SetDesc *sets, const uint2 *tiles, int ntiles, i
 // linT / lin4: the sets' parameterisation flags (one per batch), compiled in
 hipError_t launch_inscatter1(const SetDesc *sets, const uint2 *tiles, int ntiles, int nsamp,
                              bool linT, hipStream_t st);
-// G: texels per thread (2, 3, 4 or 8); CH: theta rows per staging round
-hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS, int CH, int JX,
-                             const void *dirT, const void *dirW, bool first, size_t lds, int G,
+// G: texels per thread (2, 3, 4 or 8); CH: theta rows per staging round; RNU:
+// the largest RES_NU of the launch's sets
+hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS, int CH, int RNU,
+                             int JX, const void *dirT, const void *dirW, bool first, size_t lds, int G,
                              hipStream_t st);
 hipError_t launch_irradianceN(const SetDesc *sets, const uint2 *tiles, int ntiles, int NI, int CH,
                               const void *dirs, bool first, size_t lds, hipStream_t st);

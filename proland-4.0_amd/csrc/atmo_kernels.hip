@@ -386,8 +386,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int JXr = FIRST ? 0 : JX;
     const size_t npairs = (size_t)CH * NP + (size_t)JXr;
     __shared__ int s_jxmax;
-    float4 *pairs = lds;                                   // rows of NP (+ JX) entries of PW
-    float4 *vtmp = pairs + npairs * PW;                    // [CH][NV][NT]
+    // order 2, unchunked: the entries' three float4 as planes kJ1Plane apart
+    // (16-byte stride in each: a lookup's reads spread over the LDS banks, and
+    // one address serves all three with immediate offsets); otherwise whole
+    constexpr bool PLANES = FIRST && !CHUNK;
+    constexpr int ES = PLANES ? 1 : PW;                    // entry stride (float4)
+    constexpr int CS = PLANES ? kJ1Plane : 1;              // component stride (float4)
+    if (PLANES && npairs > (size_t)kJ1Plane) return;       // the host never launches this
+    float4 *pairs = lds;                                   // rows of NP (+ JX) entries
+    float4 *vtmp = pairs + (PLANES ? 2 * kJ1Plane + npairs : npairs * PW);   // [CH][NV][NT]
     float4 *epair = vtmp + (size_t)CH * NV * NT;           // [EPAD + SW + 1][2] (order 2)
     float4 *gdat = epair + (FIRST ? (size_t)(EPAD + s.SW + 1) * 2 : 0);   // [CH][3]
 
@@ -519,7 +526,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             if (!FIRST && it >= ig0) continue;   // merged ground row (4b)
             const float4 a = vtmp[((size_t)it * NV + kk) * NT];
             const float4 b = vtmp[((size_t)it * NV + kk + 1) * NT];
-            float4 *q = pairs + ((size_t)rowoff(it) + kk) * PW;
+            float4 *q = pairs + ((size_t)rowoff(it) + kk) * ES;
             const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
             const float px = at_zero(a.x, dx, kk), py = at_zero(a.y, dy, kk), pz = at_zero(a.z, dz, kk);
             if (!FIRST) {
@@ -532,8 +539,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 const float4 bm = vtmp[((size_t)it * NV + kk + 1) * NT + 1];
                 const float ex = bm.x - am.x, ey = bm.y - am.y, ez = bm.z - am.z;
                 q[0] = make_float4(px, py, pz, dx);
-                q[1] = make_float4(dy, dz, at_zero(am.x, ex, kk), at_zero(am.y, ey, kk));
-                q[2] = make_float4(at_zero(am.z, ez, kk), ex, ey, ez);
+                q[CS] = make_float4(dy, dz, at_zero(am.x, ex, kk), at_zero(am.y, ey, kk));
+                q[2 * CS] = make_float4(at_zero(am.z, ez, kk), ex, ey, ez);
             }
         }
         // (4b) orders >= 3, ground rows: the lookup of a ground direction is
@@ -636,9 +643,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             // the caller applies q1, whose per-set constants are folded into the staging)
             auto lookup = [&](const float4 *pt, float pos, float out[3], float &q1) {
                 const unsigned kk = __float2uint_rz(pos);
-                // PW = 3: a 24-bit multiply (full rate; kk <= RNU) instead of the
-                // 32-bit one the compiler picks; PW = 2: a shift
-                const float4 *pp = pt + (FIRST ? __umul24(kk, (unsigned)PW) : kk * PW);
+                // ES = 3: a 24-bit multiply (full rate; kk <= RNU) instead of the
+                // 32-bit one the compiler picks; 2 and 1: a shift
+                const float4 *pp = pt + (ES == 3 ? __umul24(kk, (unsigned)ES) : kk * ES);
                 const float4 a = pp[0];
                 if (!FIRST) {
                     const float2 d = *(const float2 *)(pp + 1);
@@ -647,7 +654,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                     out[2] = fmaf(pos, d.y, a.z);
                     q1 = 1.0f;
                 } else {
-                    const float4 b = pp[1], c2 = pp[2];
+                    const float4 b = pp[CS], c2 = pp[2 * CS];
                     const float nu1 = fmaf(pos, iposA, nposBi);
                     q1 = fmaf(nu1, nu1, 1.0f);
                     const float y1 = __builtin_amdgcn_rsqf(fmaf(m2g, nu1, g2p1));
@@ -689,7 +696,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 const float vzwz = vz * ct;
                 const float cx0 = fmaf(m2g, vzwz, g2p1);
                 const float dw = g1.w, cp = g1.z;
-                const float4 *pt = pairs + (size_t)rowoff(itl) * PW;
+                const float4 *pt = pairs + (size_t)rowoff(itl) * ES;
                 // merged ground row: entry (floor(pos) + floor(xg)) - jlo, as an
                 // index that wraps modulo 2^32 (jlo may exceed the row offset)
                 const unsigned gbase = (unsigned)(rowoff(itl) - __float_as_int(gdat[3 * itl + 2].x)) * PW;
@@ -1414,10 +1421,14 @@ __global__ __launch_bounds__(256) void k_check(const SetDesc *__restrict__ sets,
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-size_t lds_inscatterS(int CH, int RNU, int SW, int JX, bool first) {
+bool inscatterS_chunked(int NS, int CH, int RNU, bool first) {
+    return CH < NS || (first && (size_t)CH * (RNU + 1) > (size_t)kJ1Plane);
+}
+size_t lds_inscatterS(int NS, int CH, int RNU, int SW, int JX, bool first) {
     const int NT = first ? 2 : 1, PW = first ? 3 : 2;
     const size_t npairs = (size_t)CH * (RNU + 1) + (first ? 0 : (size_t)JX);
-    return sizeof(float4) * (npairs * PW + (size_t)CH * (RNU + 2) * NT +
+    const bool planes = first && !inscatterS_chunked(NS, CH, RNU, first);
+    return sizeof(float4) * ((planes ? 2 * (size_t)kJ1Plane + npairs : npairs * PW) + (size_t)CH * (RNU + 2) * NT +
                              (first ? (size_t)(e_pad(SW) + SW + 1) * 2 : 0) + (size_t)CH * 3);
 }
 int merged_row_extra(double r, double Rg, double Rt, int SW, int NS, int it) {
@@ -1442,7 +1453,7 @@ size_t lds_irradianceN(int CH, int RNU, bool first) {
 }
 int chunk_inscatterS(int NS, int RNU, int SW, const std::function<int(int)> &jx_of_ch, bool first) {
     for (int ch = NS; ch >= 1; --ch)
-        if (lds_inscatterS(ch, RNU, SW, first ? 0 : jx_of_ch(ch), first) <= kSphLdsBudget) return ch;
+        if (lds_inscatterS(NS, ch, RNU, SW, first ? 0 : jx_of_ch(ch), first) <= kSphLdsBudget) return ch;
     return 0;
 }
 int chunk_irradianceN(int NI, int RNU, bool first) {
@@ -1482,8 +1493,8 @@ hipError_t launch_inscatter1(const SetDesc *sets, const uint2 *tiles, int ntiles
         hipLaunchKernelGGL(k_inscatter1<false>, dim3(ntiles), dim3(256), 0, st, sets, tiles, nsamp);
     return hipGetLastError();
 }
-hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS, int CH, int JX,
-                             const void *dirT, const void *dirW, bool first, size_t lds, int G,
+hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles, int NS, int CH, int RNU,
+                             int JX, const void *dirT, const void *dirW, bool first, size_t lds, int G,
                              hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
     if (CH < 1 || CH > NS || JX < 0) return hipErrorInvalidValue;
@@ -1491,7 +1502,7 @@ hipError_t launch_inscatterS(const SetDesc *sets, const uint2 *tiles, int ntiles
     hipLaunchKernelGGL((k_inscatterS<F, GG, C>), dim3(ntiles), dim3(256), lds, st, sets, tiles, NS, \
                        CH, JX, (const DirJ *)dirT, (const float2 *)dirW)
     if (G != 2 && G != 3 && G != 4 && G != 8) return hipErrorInvalidValue;
-    if (CH < NS) {   // chunked staging: one variant (all G give the same bits)
+    if (inscatterS_chunked(NS, CH, RNU, first)) {   // chunked staging: one variant (all G give the same bits)
         if (first) ATMO_LJ(true, 4, true); else ATMO_LJ(false, 4, true);
     } else if (G == 2) {
         if (first) ATMO_LJ(true, 2, false); else ATMO_LJ(false, 2, false);
