@@ -126,7 +126,7 @@ struct S1Entry {
     float T[3];      // T(r, mu, t_i), trapezoid weight folded in
     float eR, eM;    // exp(-h_i/HR), exp(-h_i/HM)
     float b;         // T-table row weight of r_i
-    int j0, j1;      // T-table row offsets (texels) of r_i
+    unsigned j0, j1; // T-table row offsets (bytes) of r_i
 };
 
 constexpr int kS1Chunk = 64;
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(256) void k_inscatter1(const SetDesc *__restrict__ 
         }
     }
     const float iHR = (float)(1.0 / s.HR), iHM = (float)(1.0 / s.HM);
-    const gfloat4 *T4 = gptr(s.T4);
+    const __amdgpu_buffer_rsrc_t Tr = table_rsrc(s.T4, (unsigned)((size_t)s.TW * s.TH * sizeof(float4)));
     for (int c0 = 0; c0 <= nsamp; c0 += kS1Chunk) {
         const int nc = min(kS1Chunk, nsamp + 1 - c0);
         __syncthreads();
@@ -220,8 +220,8 @@ __global__ __launch_bounds__(256) void k_inscatter1(const SetDesc *__restrict__ 
             double bd;
             gl_axis_dd(T_ur_d(s, en.q / (rid + s.Rg)) * s.TH - 0.5, s.TH, j0, j1, bd);
             en.b = (float)bd;
-            en.j0 = j0 * s.TW;
-            en.j1 = j1 * s.TW;
+            en.j0 = (unsigned)(j0 * s.TW) * 16u;
+            en.j1 = (unsigned)(j1 * s.TW) * 16u;
             ent[row][js] = en;
         }
         __syncthreads();
@@ -257,8 +257,10 @@ __global__ __launch_bounds__(256) void k_inscatter1(const SetDesc *__restrict__ 
                 } else {   // more than kXkMax columns: the FP32 coordinate
                     gl_axis(T_umu_t<false>(s, (float)mus) * s.TW - 0.5f, s.TW, i0, i1, a);
                 }
-                const float3 t00 = ld3(T4 + en.j0 + i0), t01 = ld3(T4 + en.j0 + i1);
-                const float3 t10 = ld3(T4 + en.j1 + i0), t11 = ld3(T4 + en.j1 + i1);
+                // buffer loads at byte offsets (i0, i1 >= 0: clamped to the table)
+                const unsigned o0 = (unsigned)i0 << 4, o1 = (unsigned)i1 << 4;
+                const float3 t00 = ld3b(Tr, en.j0 + o0, 0u), t01 = ld3b(Tr, en.j0 + o1, 0u);
+                const float3 t10 = ld3b(Tr, en.j1 + o0, 0u), t11 = ld3b(Tr, en.j1 + o1, 0u);
                 const float3 Ts = lerp3(lerp3(t00, t01, a), lerp3(t10, t11, a), en.b);
                 const float tx = en.T[0] * Ts.x, ty = en.T[1] * Ts.y, tz = en.T[2] * Ts.z;
                 r0 = fmaf(tx, en.eR, r0); r1 = fmaf(ty, en.eR, r1); r2 = fmaf(tz, en.eR, r2);
