@@ -396,8 +396,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     constexpr int CS = PLANES ? kJ1Plane : 1;              // component stride (float4)
     if (PLANES && npairs > (size_t)kJ1Plane) return;       // the host never launches this
     float4 *pairs = lds;                                   // rows of NP (+ JX) entries
-    float4 *vtmp = pairs + (PLANES ? 2 * kJ1Plane + npairs : npairs * PW);   // [CH][NV][NT]
-    float4 *epair = vtmp + (size_t)CH * NV * NT;           // [EPAD + SW + 1][2] (order 2)
+    // [CH][NV][NT] slice values (the planes compute each entry's two directly)
+    float4 *vtmp = pairs + (PLANES ? 2 * kJ1Plane + npairs : npairs * PW);
+    float4 *epair = vtmp + (PLANES ? 0 : (size_t)CH * NV * NT);   // [EPAD + SW + 1][2] (order 2)
     float4 *gdat = epair + (FIRST ? (size_t)(EPAD + s.SW + 1) * 2 : 0);   // [CH][3]
 
     const LayerGeom L = s.layers[l];
@@ -438,18 +439,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const bool last = !CHUNK || c0 + nth == NS;
         if (CHUNK && c0 > 0) __syncthreads();   // every thread is done with the previous chunk's rows
         // (1) slice values V[it][kk], kk = slice + 1
-        for (int e = threadIdx.x; e < nth * NV; e += blockDim.x) {
-            const int it = c0 + e / NV, kk = e % NV;
+        auto slice_value = [&](int it, int kk, float4 v[NT]) {
             const double ct = cos((it + 0.5) * (kPI / NS));
             const double rmu = r * ct;
             double tz, ty;
             tex4d_rmu(s, rho2, rmu, rmu * rmu - rho2, ct, tz, ty);
-            float4 v[NT];
             stage_slice<NT>(s, tabs, tz, ty, txs, kk - 1, v);
             if (FIRST) {  // fold the phase-function constants in
                 v[0].x *= s.kR; v[0].y *= s.kR; v[0].z *= s.kR;
                 v[NT - 1].x *= s.kM; v[NT - 1].y *= s.kM; v[NT - 1].z *= s.kM;
             }
+        };
+        for (int e = threadIdx.x; !PLANES && e < nth * NV; e += blockDim.x) {
+            float4 v[NT];
+            slice_value(c0 + e / NV, e % NV, v);
     #pragma unroll
             for (int n = 0; n < NT; ++n) vtmp[(size_t)e * NT + n] = v[n];
         }
@@ -526,8 +529,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         for (int e = threadIdx.x; e < nth * NP; e += blockDim.x) {
             const int it = e / NP, kk = e % NP;
             if (!FIRST && it >= ig0) continue;   // merged ground row (4b)
-            const float4 a = vtmp[((size_t)it * NV + kk) * NT];
-            const float4 b = vtmp[((size_t)it * NV + kk + 1) * NT];
+            float4 va[NT], vb[NT];   // V[kk-1], V[kk]
+            if (PLANES) {            // computed here: no slice-value staging (LDS)
+                slice_value(c0 + it, kk, va);
+                slice_value(c0 + it, kk + 1, vb);
+            } else {
+    #pragma unroll
+                for (int n = 0; n < NT; ++n) {
+                    va[n] = vtmp[((size_t)it * NV + kk) * NT + n];
+                    vb[n] = vtmp[((size_t)it * NV + kk + 1) * NT + n];
+                }
+            }
+            const float4 a = va[0], b = vb[0];
             float4 *q = pairs + ((size_t)rowoff(it) + kk) * ES;
             const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
             const float px = at_zero(a.x, dx, kk), py = at_zero(a.y, dy, kk), pz = at_zero(a.z, dz, kk);
@@ -537,8 +550,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 q[1] = make_float4(dy, dz, 0.f, 0.f);
             } else {
                 // Rayleigh and Mie pairs in 48 bytes: three 16-byte LDS reads
-                const float4 am = vtmp[((size_t)it * NV + kk) * NT + 1];
-                const float4 bm = vtmp[((size_t)it * NV + kk + 1) * NT + 1];
+                const float4 am = va[NT - 1], bm = vb[NT - 1];
                 const float ex = bm.x - am.x, ey = bm.y - am.y, ez = bm.z - am.z;
                 q[0] = make_float4(px, py, pz, dx);
                 q[CS] = make_float4(dy, dz, at_zero(am.x, ex, kk), at_zero(am.y, ey, kk));
@@ -1430,7 +1442,8 @@ size_t lds_inscatterS(int NS, int CH, int RNU, int SW, int JX, bool first) {
     const int NT = first ? 2 : 1, PW = first ? 3 : 2;
     const size_t npairs = (size_t)CH * (RNU + 1) + (first ? 0 : (size_t)JX);
     const bool planes = first && !inscatterS_chunked(NS, CH, RNU, first);
-    return sizeof(float4) * ((planes ? 2 * (size_t)kJ1Plane + npairs : npairs * PW) + (size_t)CH * (RNU + 2) * NT +
+    return sizeof(float4) * ((planes ? 2 * (size_t)kJ1Plane + npairs : npairs * PW) +
+                             (planes ? 0 : (size_t)CH * (RNU + 2) * NT) +
                              (first ? (size_t)(e_pad(SW) + SW + 1) * 2 : 0) + (size_t)CH * 3);
 }
 int merged_row_extra(double r, double Rg, double Rt, int SW, int NS, int it) {
