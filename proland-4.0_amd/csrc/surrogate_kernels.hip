@@ -570,10 +570,11 @@ constexpr int kRowPad = 264;   // bf16 per staged weight row (512 B + 16 B pad)
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
+// a in the low half, b in the high half (one v_cvt_pk_bf16_f32, round to nearest even)
 __device__ __forceinline__ unsigned pack_bf16(float a, float b) {
-    const unsigned lo = __builtin_bit_cast(unsigned short, (__bf16)a);
-    const unsigned hi = __builtin_bit_cast(unsigned short, (__bf16)b);
-    return lo | (hi << 16);
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){a, b}, bf16x2));
 }
 
 // D tile (features x 32 rows) -> act(D + bias) as the B operands of k blocks
@@ -586,28 +587,29 @@ __device__ __forceinline__ void tile_to_b(const f32x16 &acc, const float *bias, 
         float a[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const float z = acc[4 * q + t] + (bias ? bias[8 * q + 4 * h + t] : 0.0f);
+            // (no "+ 0.0f" without a bias: x + 0 is not x for x = -0, so it would be kept)
+            const float z = bias ? acc[4 * q + t] + bias[8 * q + 4 * h + t] : acc[4 * q + t];
             a[t] = act(z);
         }
         P[q][0] = pack_bf16(a[0], a[1]);
         P[q][1] = pack_bf16(a[2], a[3]);
     }
     // lane h = 0 keeps features 0-3, 16-19 and needs 4-7, 20-23 from lane l^32;
-    // lane h = 1 keeps 12-15, 28-31 and needs 8-11, 24-27
-    unsigned ra[2], rb[2];
+    // lane h = 1 keeps 12-15, 28-31 and needs 8-11, 24-27.  v_permlane32_swap
+    // exchanges the upper 32 lanes of its first operand with the lower 32 lanes
+    // of its second: swapping (P[0], P[1]) leaves lanes h = 0 with their own
+    // P[0] and their partner's P[0], lanes h = 1 with their partner's P[1] and
+    // their own P[1], i.e. features 8h .. 8h + 7 in order in both halves (and
+    // (P[2], P[3]) the same for 16 + 8h ..), with no lane-dependent selects.
+    unsigned X[2], Y[2], Z[2], W[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-        ra[i] = (unsigned)__shfl_xor((int)(h ? P[0][i] : P[1][i]), 32, 64);
-        rb[i] = (unsigned)__shfl_xor((int)(h ? P[2][i] : P[3][i]), 32, 64);
+        const auto a = __builtin_amdgcn_permlane32_swap(P[0][i], P[1][i], false, false);
+        const auto b = __builtin_amdgcn_permlane32_swap(P[2][i], P[3][i], false, false);
+        X[i] = a[0]; Y[i] = a[1]; Z[i] = b[0]; W[i] = b[1];
     }
-    u32x4 L, Hh;
-    if (h == 0) {
-        L = u32x4{P[0][0], P[0][1], ra[0], ra[1]};
-        Hh = u32x4{P[2][0], P[2][1], rb[0], rb[1]};
-    } else {
-        L = u32x4{ra[0], ra[1], P[1][0], P[1][1]};
-        Hh = u32x4{rb[0], rb[1], P[3][0], P[3][1]};
-    }
+    const u32x4 L = u32x4{X[0], X[1], Y[0], Y[1]};
+    const u32x4 Hh = u32x4{Z[0], Z[1], W[0], W[1]};
     lo = __builtin_bit_cast(bf16x8, L);
     hi = __builtin_bit_cast(bf16x8, Hh);
 }
@@ -632,14 +634,20 @@ __device__ __forceinline__ void stage_half_async(const __bf16 *W, __bf16 *wl, in
 
 // Output tiles nt0 .. nt0+3 of one hidden layer from the staged weight rows,
 // software-pipelined: the matrix-core products of tile nt run while the
-// vector units apply bias + act and relayout tile nt - 1.
+// vector units apply act and relayout tile nt - 1.  The accumulators start at
+// the tile's biases (element 4q + t of lane (c, h) is feature 8q + 4h + t).
 template <int RBW>
-__device__ __forceinline__ void tile_mfma(const __bf16 *wl, int nt, int c, int h, const bf16x8 (&H)[RBW][16],
-                                          f32x16 (&acc)[RBW]) {
+__device__ __forceinline__ void tile_mfma(const __bf16 *wl, const float *bias, int nt, int c, int h,
+                                          const bf16x8 (&H)[RBW][16], f32x16 (&acc)[RBW]) {
+    const float4 *bq = (const float4 *)(bias + 32 * nt + 4 * h);
 #pragma unroll
-    for (int rb = 0; rb < RBW; ++rb)
+    for (int q = 0; q < 4; ++q) {
+        const float4 b = bq[2 * q];
 #pragma unroll
-        for (int j = 0; j < 16; ++j) acc[rb][j] = 0.0f;
+        for (int rb = 0; rb < RBW; ++rb) {
+            acc[rb][4 * q] = b.x; acc[rb][4 * q + 1] = b.y; acc[rb][4 * q + 2] = b.z; acc[rb][4 * q + 3] = b.w;
+        }
+    }
     const __bf16 *wrow = wl + (32 * nt + c) * kRowPad + 8 * h;
 #pragma unroll
     for (int kb = 0; kb < 16; ++kb) {
@@ -654,14 +662,14 @@ template <int RBW>
 __device__ __forceinline__ void hidden_half(const __bf16 *wl, const float *bias, int nt0, int c, int h,
                                             const bf16x8 (&H)[RBW][16], bf16x8 (&Hn)[RBW][16]) {
     f32x16 acc[2][RBW];
-    tile_mfma<RBW>(wl, nt0, c, h, H, acc[0]);
+    tile_mfma<RBW>(wl, bias, nt0, c, h, H, acc[0]);
 #pragma unroll
     for (int i = 1; i <= 4; ++i) {
-        if (i < 4) tile_mfma<RBW>(wl, nt0 + i, c, h, H, acc[i & 1]);
+        if (i < 4) tile_mfma<RBW>(wl, bias, nt0 + i, c, h, H, acc[i & 1]);
         const int nt = nt0 + i - 1;
 #pragma unroll
         for (int rb = 0; rb < RBW; ++rb)
-            tile_to_b(acc[(i - 1) & 1][rb], bias + 32 * nt, h, Hn[rb][2 * nt], Hn[rb][2 * nt + 1]);
+            tile_to_b(acc[(i - 1) & 1][rb], nullptr, h, Hn[rb][2 * nt], Hn[rb][2 * nt + 1]);
     }
 }
 
