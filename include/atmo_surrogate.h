@@ -14,9 +14,11 @@
  * outputs (log-scaled rgb of the contribution of the orders above the exactly
  * computed ones, standardised; the 4th output is trained to 0 because those
  * orders add nothing to alpha).
- * The two 256x256 layers run in BF16 on the matrix cores with FP32
+ * Training runs the two 256x256 layers in BF16 on the matrix cores with FP32
  * accumulation; master weights, Adam state, the first and last layers are
- * FP32.
+ * FP32.  The fused inference kernel runs the hidden and output layers in FP16
+ * on the matrix cores (FP32 accumulation, FP16 activations) and the first
+ * layer as a BF16 high + low pair.
  *
  * Parameter vector layout (FP32): W1[14][256], b1[256], then for l = 2..3
  * Wl[256][256] (row = input), bl[256], then the output layer W5[256][4],

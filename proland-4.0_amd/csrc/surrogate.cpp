@@ -158,18 +158,18 @@ atmo_status ensure_cap(atmo_surrogate *s, int rows, bool train) {
     return ATMO_OK;
 }
 
-// fused-inference weight buffer: a1h, a1l [256][16], a5h, a5l [4][256] (bf16), hidden biases
-// [kNG][256] (FP32)
+// fused-inference weight buffer: a1h, a1l [256][16] (bf16), a5h, a5l [4][256] (FP16), hidden
+// biases [kNG][256] (FP32), hidden weights [kNG][256][256] as [out][in] (FP16)
 constexpr size_t FZ_A1H = 0, FZ_A1L = FZ_A1H + kNH * 16 * 2, FZ_A5H = FZ_A1L + kNH * 16 * 2,
                  FZ_A5L = FZ_A5H + kNOUT * kNH * 2, FZ_BH = FZ_A5L + kNOUT * kNH * 2,
-                 FZ_BYTES = FZ_BH + kNG * kNH * 4;
+                 FZ_W16 = FZ_BH + kNG * kNH * 4, FZ_BYTES = FZ_W16 + (size_t)kNG * kNH * kNH * 2;
 
 FusedWeights fused_weights(atmo_surrogate *s) {
     char *z = (char *)s->fz.p;
     const float *P = s->p.as<float>();
-    return FusedWeights{(const __bf16 *)(z + FZ_A1H), (const __bf16 *)(z + FZ_A1L), s->Wt(2),
-                        (const float *)(z + FZ_BH), (const __bf16 *)(z + FZ_A5H), (const __bf16 *)(z + FZ_A5L),
-                        P + OFF_B5};
+    return FusedWeights{(const __bf16 *)(z + FZ_A1H), (const __bf16 *)(z + FZ_A1L), (const _Float16 *)(z + FZ_W16),
+                        (const float *)(z + FZ_BH), (const _Float16 *)(z + FZ_A5H),
+                        (const _Float16 *)(z + FZ_A5L), P + OFF_B5};
 }
 
 atmo_status refresh_bf16(atmo_surrogate *s) {
@@ -177,9 +177,9 @@ atmo_status refresh_bf16(atmo_surrogate *s) {
         HIPCHK(to_bf16(s->p.as<float>() + offW(L), s->Wb(L), s->Wt(L), s->st));
     const float *P = s->p.as<float>();
     char *z = (char *)s->fz.p;
-    HIPCHK(prep_fused(P + OFF_W1, P + OFF_B1, P + OFF_W5, P + offB(2), LAYER_SZ, (__bf16 *)(z + FZ_A1H),
-                      (__bf16 *)(z + FZ_A1L), (__bf16 *)(z + FZ_A5H), (__bf16 *)(z + FZ_A5L),
-                      (float *)(z + FZ_BH), s->st));
+    HIPCHK(prep_fused(P + OFF_W1, P + OFF_B1, P + OFF_W5, P + offW(2), P + offB(2), LAYER_SZ,
+                      (__bf16 *)(z + FZ_A1H), (__bf16 *)(z + FZ_A1L), (_Float16 *)(z + FZ_W16),
+                      (_Float16 *)(z + FZ_A5H), (_Float16 *)(z + FZ_A5L), (float *)(z + FZ_BH), s->st));
     return ATMO_OK;
 }
 

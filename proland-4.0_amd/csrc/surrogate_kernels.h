@@ -34,11 +34,11 @@ struct TargetNorm {
 
 // Weights of the fused inference kernel (device pointers), from prep_fused.
 struct FusedWeights {
-    const __bf16 *a1h, *a1l;  // [256][16] layer 1 (column 14: bias), bf16 hi / lo
-    const __bf16 *wt;         // [kNG][256][256] layers 2..kNL as [out][in]
-    const float *bh;          // [kNG][256] biases of layers 2..kNL
-    const __bf16 *a5h, *a5l;  // [4][256] layer 5 as [out][in], bf16 hi / lo
-    const float *b5;          // [4]
+    const __bf16 *a1h, *a1l;     // [256][16] layer 1 (column 14: bias), bf16 hi / lo
+    const _Float16 *wt;          // [kNG][256][256] layers 2..kNL as [out][in], FP16
+    const float *bh;             // [kNG][256] biases of layers 2..kNL
+    const _Float16 *a5h, *a5l;   // [4][256] layer 5 as [out][in], FP16 hi / lo
+    const float *b5;             // [4]
 };
 
 // One set of a fused inference launch: its features, its first workgroup
@@ -75,9 +75,11 @@ hipError_t code_rows(const FeatSet &fs, const unsigned *codes, int n, const floa
 // out[6] (device, zeroed by the caller): sums and sums of squares of the rgb targets
 hipError_t target_stats(const float *Tg, size_t n, double *out, hipStream_t st);
 hipError_t norm_targets(float *Tg, size_t n, const TargetNorm &tn, hipStream_t st);
-// bh0: bias of layer 2, the biases of layers 3..kNL follow at stride bstride floats
-hipError_t prep_fused(const float *W1, const float *b1, const float *W5, const float *bh0, size_t bstride,
-                      __bf16 *a1h, __bf16 *a1l, __bf16 *a5h, __bf16 *a5l, float *bh, hipStream_t st);
+// Wh0 / bh0: weight ([in][out]) and bias of layer 2, those of layers 3..kNL follow
+// at stride bstride floats
+hipError_t prep_fused(const float *W1, const float *b1, const float *W5, const float *Wh0, const float *bh0,
+                      size_t bstride, __bf16 *a1h, __bf16 *a1l, _Float16 *wt, _Float16 *a5h, _Float16 *a5l,
+                      float *bh, hipStream_t st);
 // Fused inference of every S and E texel of a batch of sets (device array
 // `sets`, tile0 ascending, ntiles workgroups in all): adds the predicted
 // higher orders to S.rgb and E.

@@ -15,6 +15,7 @@ namespace atmo {
 namespace sg {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 f16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
 // Hidden activation: z gated by a smoothstep, act(z) = z s(t), t = clamp(z / 6
@@ -570,29 +571,37 @@ constexpr int kRowPad = 264;   // bf16 per staged weight row (512 B + 16 B pad)
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-// a in the low half, b in the high half (one v_cvt_pk_bf16_f32, round to nearest even)
-__device__ __forceinline__ unsigned pack_bf16(float a, float b) {
+// act of a (low half) and b (high half) in packed FP16: one v_cvt_pk_f16_f32
+// (round to nearest even) and five v_pk_*_f16, the clamp of t riding on the
+// first FMA.  FP16 keeps 11 significant bits where the bf16 activations of the
+// training path keep 8, and its range covers the pre-activations (|z| < 65504).
+__device__ __forceinline__ unsigned act_pk(float a, float b) {
     typedef float f32x2 __attribute__((ext_vector_type(2)));
-    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-    return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){a, b}, bf16x2));
+    typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+    const f16x2 z = __builtin_convertvector((f32x2){a, b}, f16x2);
+    const _Float16 k6 = (_Float16)(1.0f / 6.0f);
+    const f16x2 c6 = {k6, k6}, c05 = {(_Float16)0.5f, (_Float16)0.5f};
+    const f16x2 zero = {(_Float16)0.0f, (_Float16)0.0f}, one = {(_Float16)1.0f, (_Float16)1.0f};
+    const f16x2 m2 = {(_Float16)-2.0f, (_Float16)-2.0f}, three = {(_Float16)3.0f, (_Float16)3.0f};
+    f16x2 t = __builtin_elementwise_fma(z, c6, c05);
+    t = __builtin_elementwise_min(__builtin_elementwise_max(t, zero), one);
+    const f16x2 u = __builtin_elementwise_fma(m2, t, three);
+    return __builtin_bit_cast(unsigned, z * (t * t * u));
 }
 
-// D tile (features x 32 rows) -> act(D + bias) as the B operands of k blocks
-// 2nt (lo) and 2nt + 1 (hi).  bias: the tile's 32 biases or nullptr.
-__device__ __forceinline__ void tile_to_b(const f32x16 &acc, const float *bias, int h, bf16x8 &lo,
-                                          bf16x8 &hi) {
+// D tile (features x 32 rows) -> act(D + bias) as the FP16 B operands of k
+// blocks 2nt (lo) and 2nt + 1 (hi).  bias: the tile's 32 biases or nullptr.
+__device__ __forceinline__ void tile_to_b(const f32x16 &acc, const float *bias, int h, f16x8 &lo,
+                                          f16x8 &hi) {
     unsigned P[4][2];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        float a[4];
+        float z[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            // (no "+ 0.0f" without a bias: x + 0 is not x for x = -0, so it would be kept)
-            const float z = bias ? acc[4 * q + t] + bias[8 * q + 4 * h + t] : acc[4 * q + t];
-            a[t] = act(z);
-        }
-        P[q][0] = pack_bf16(a[0], a[1]);
-        P[q][1] = pack_bf16(a[2], a[3]);
+        for (int t = 0; t < 4; ++t)   // (no "+ 0.0f" without a bias: it would be kept, -0 + 0 = +0)
+            z[t] = bias ? acc[4 * q + t] + bias[8 * q + 4 * h + t] : acc[4 * q + t];
+        P[q][0] = act_pk(z[0], z[1]);
+        P[q][1] = act_pk(z[2], z[3]);
     }
     // lane h = 0 keeps features 0-3, 16-19 and needs 4-7, 20-23 from lane l^32;
     // lane h = 1 keeps 12-15, 28-31 and needs 8-11, 24-27.  v_permlane32_swap
@@ -610,8 +619,8 @@ __device__ __forceinline__ void tile_to_b(const f32x16 &acc, const float *bias, 
     }
     const u32x4 L = u32x4{X[0], X[1], Y[0], Y[1]};
     const u32x4 Hh = u32x4{Z[0], Z[1], W[0], W[1]};
-    lo = __builtin_bit_cast(bf16x8, L);
-    hi = __builtin_bit_cast(bf16x8, Hh);
+    lo = __builtin_bit_cast(f16x8, L);
+    hi = __builtin_bit_cast(f16x8, Hh);
 }
 
 typedef __attribute__((address_space(3))) void lds_void;
@@ -620,7 +629,7 @@ typedef __attribute__((address_space(3))) void lds_void;
 // the padded LDS image (global_load_lds_dwordx4: each active lane moves 16 B
 // straight into LDS; one row = 32 lanes).  Each of the NW waves moves 128/NW rows.
 template <int NW>
-__device__ __forceinline__ void stage_half_async(const __bf16 *W, __bf16 *wl, int r0, int wv, int lane) {
+__device__ __forceinline__ void stage_half_async(const _Float16 *W, _Float16 *wl, int r0, int wv, int lane) {
     constexpr int RPWV = 128 / NW;
     if (lane < 32) {
 #pragma unroll 8
@@ -637,8 +646,8 @@ __device__ __forceinline__ void stage_half_async(const __bf16 *W, __bf16 *wl, in
 // vector units apply act and relayout tile nt - 1.  The accumulators start at
 // the tile's biases (element 4q + t of lane (c, h) is feature 8q + 4h + t).
 template <int RBW>
-__device__ __forceinline__ void tile_mfma(const __bf16 *wl, const float *bias, int nt, int c, int h,
-                                          const bf16x8 (&H)[RBW][16], f32x16 (&acc)[RBW]) {
+__device__ __forceinline__ void tile_mfma(const _Float16 *wl, const float *bias, int nt, int c, int h,
+                                          const f16x8 (&H)[RBW][16], f32x16 (&acc)[RBW]) {
     const float4 *bq = (const float4 *)(bias + 32 * nt + 4 * h);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -648,19 +657,19 @@ __device__ __forceinline__ void tile_mfma(const __bf16 *wl, const float *bias, i
             acc[rb][4 * q] = b.x; acc[rb][4 * q + 1] = b.y; acc[rb][4 * q + 2] = b.z; acc[rb][4 * q + 3] = b.w;
         }
     }
-    const __bf16 *wrow = wl + (32 * nt + c) * kRowPad + 8 * h;
+    const _Float16 *wrow = wl + (32 * nt + c) * kRowPad + 8 * h;
 #pragma unroll
     for (int kb = 0; kb < 16; ++kb) {
-        const bf16x8 a = *(const bf16x8 *)(wrow + 16 * kb);
+        const f16x8 a = *(const f16x8 *)(wrow + 16 * kb);
 #pragma unroll
         for (int rb = 0; rb < RBW; ++rb)
-            acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, H[rb][kb], acc[rb], 0, 0, 0);
+            acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, H[rb][kb], acc[rb], 0, 0, 0);
     }
 }
 
 template <int RBW>
-__device__ __forceinline__ void hidden_half(const __bf16 *wl, const float *bias, int nt0, int c, int h,
-                                            const bf16x8 (&H)[RBW][16], bf16x8 (&Hn)[RBW][16]) {
+__device__ __forceinline__ void hidden_half(const _Float16 *wl, const float *bias, int nt0, int c, int h,
+                                            const f16x8 (&H)[RBW][16], f16x8 (&Hn)[RBW][16]) {
     f32x16 acc[2][RBW];
     tile_mfma<RBW>(wl, bias, nt0, c, h, H, acc[0]);
 #pragma unroll
@@ -686,7 +695,7 @@ __device__ __forceinline__ void wait_all_and_sync() {
 template <int RBW, int NW>
 __global__ __launch_bounds__(64 * NW) void k_mlp_infer(const InferSet *__restrict__ sets, int nsets,
                                                        FusedWeights fw, TargetNorm tn) {
-    extern __shared__ __attribute__((aligned(16))) __bf16 wl[];   // [256][kRowPad]
+    extern __shared__ __attribute__((aligned(16))) _Float16 wl[];   // [256][kRowPad] (layer 1: bf16 bits)
     __shared__ float bs[kNG][kNH];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, h = lane >> 5, c = lane & 31;
     // the set of this workgroup: last set whose first tile <= blockIdx.x
@@ -701,7 +710,7 @@ __global__ __launch_bounds__(64 * NW) void k_mlp_infer(const InferSet *__restric
     float4 *__restrict__ S = is.S;
     float *__restrict__ E = is.E;
     const int rw = ((int)blockIdx.x - is.tile0) * (32 * RBW * NW) + wv * (32 * RBW);    // first row of this wave
-    __bf16 *wB = wl + 128 * kRowPad;
+    _Float16 *wB = wl + 128 * kRowPad;
 
     // layer-1 weights (hi then lo, [256][16]) into B, hidden biases; layer 2's first half -> A
     for (int i = threadIdx.x; i < 1024; i += 64 * NW)
@@ -710,7 +719,7 @@ __global__ __launch_bounds__(64 * NW) void k_mlp_infer(const InferSet *__restric
     __syncthreads();
     stage_half_async<NW>(fw.wt, wl, 0, wv, lane);
 
-    bf16x8 H[RBW][16];
+    f16x8 H[RBW][16];
     // (1) features and layer 1
 #pragma unroll
     for (int rb = 0; rb < RBW; ++rb) {
@@ -734,7 +743,7 @@ __global__ __launch_bounds__(64 * NW) void k_mlp_infer(const InferSet *__restric
         }
 #pragma unroll
         for (int nt = 0; nt < 8; ++nt) {
-            const bf16x8 ah = *(const bf16x8 *)(wB + (32 * nt + c) * 16 + 8 * h);
+            const bf16x8 ah = *(const bf16x8 *)(wB + (32 * nt + c) * 16 + 8 * h);   // bf16 bits
             const bf16x8 al = *(const bf16x8 *)(wB + 4096 + (32 * nt + c) * 16 + 8 * h);
             f32x16 acc;
 #pragma unroll
@@ -748,10 +757,10 @@ __global__ __launch_bounds__(64 * NW) void k_mlp_infer(const InferSet *__restric
     // (2) hidden layers 2..kNL
 #pragma unroll 1
     for (int L = 0; L < kNG; ++L) {
-        const __bf16 *W = fw.wt + (size_t)L * kNH * kNH;
+        const _Float16 *W = fw.wt + (size_t)L * kNH * kNH;
         wait_all_and_sync();                          // A holds rows 0..127; B is free
         stage_half_async<NW>(W, wl, 128, wv, lane);   // B <- rows 128..255
-        bf16x8 Hn[RBW][16];
+        f16x8 Hn[RBW][16];
         hidden_half<RBW>(wl, bs[L], 0, c, h, H, Hn);
         wait_all_and_sync();                          // B holds rows 128..255; A is free
         if (L < kNG - 1) {
@@ -775,16 +784,16 @@ __global__ __launch_bounds__(64 * NW) void k_mlp_infer(const InferSet *__restric
         for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
 #pragma unroll
         for (int kb = 0; kb < 16; ++kb) {
-            bf16x8 ah, al;
+            f16x8 ah, al;
             if (c < kNOUT) {
-                ah = *(const bf16x8 *)(wl + c * kNH + 16 * kb + 8 * h);
-                al = *(const bf16x8 *)(wl + 1024 + c * kNH + 16 * kb + 8 * h);
+                ah = *(const f16x8 *)(wl + c * kNH + 16 * kb + 8 * h);
+                al = *(const f16x8 *)(wl + 1024 + c * kNH + 16 * kb + 8 * h);
             } else {
 #pragma unroll
-                for (int i = 0; i < 8; ++i) ah[i] = al[i] = (__bf16)0.0f;
+                for (int i = 0; i < 8; ++i) ah[i] = al[i] = (_Float16)0.0f;
             }
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, H[rb][kb], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, H[rb][kb], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, H[rb][kb], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, H[rb][kb], acc, 0, 0, 0);
         }
         // lanes h = 0 hold outputs 0..3 of row c in acc[0..3]
         const int rr = rw + 32 * rb + c;
@@ -807,11 +816,13 @@ __global__ __launch_bounds__(64 * NW) void k_mlp_infer(const InferSet *__restric
 }
 
 // Weights of the fused kernel from the FP32 parameters: layer 1 (+ bias in
-// column 14) and layer 5 split into bf16 hi/lo, the hidden biases packed.
+// column 14) split into bf16 hi/lo, layer 5 into FP16 hi/lo, the hidden
+// weights as FP16 [out][in] and their biases packed.
 __global__ __launch_bounds__(256) void k_prep_fused(const float *__restrict__ W1, const float *__restrict__ b1,
-                                                    const float *__restrict__ W5, const float *__restrict__ bh0,
-                                                    size_t bstride, __bf16 *a1h, __bf16 *a1l, __bf16 *a5h,
-                                                    __bf16 *a5l, float *bh) {
+                                                    const float *__restrict__ W5, const float *__restrict__ Wh0,
+                                                    const float *__restrict__ bh0, size_t bstride, __bf16 *a1h,
+                                                    __bf16 *a1l, _Float16 *wt, _Float16 *a5h, _Float16 *a5l,
+                                                    float *bh) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < kNH * 16) {
         const int nn = i / 16, k = i % 16;
@@ -822,12 +833,16 @@ __global__ __launch_bounds__(256) void k_prep_fused(const float *__restrict__ W1
     } else if (i < kNH * 16 + kNOUT * kNH) {
         const int j = i - kNH * 16, o = j / kNH, k = j % kNH;
         const float v = W5[k * kNOUT + o];
-        const __bf16 vh = (__bf16)v;
+        const _Float16 vh = (_Float16)v;
         a5h[j] = vh;
-        a5l[j] = (__bf16)(v - (float)vh);
+        a5l[j] = (_Float16)(v - (float)vh);
     } else if (i < kNH * 16 + kNOUT * kNH + kNG * kNH) {
         const int j = i - kNH * 16 - kNOUT * kNH;
         bh[j] = bh0[(size_t)(j / kNH) * bstride + j % kNH];
+    } else if (i < kNH * 16 + kNOUT * kNH + kNG * kNH + kNG * kNH * kNH) {
+        const int j = i - kNH * 16 - kNOUT * kNH - kNG * kNH;   // layer L, output n, input k
+        const int L = j / (kNH * kNH), n = (j / kNH) % kNH, k = j % kNH;
+        wt[j] = (_Float16)Wh0[(size_t)L * bstride + (size_t)k * kNH + n];
     }
 }
 
@@ -931,11 +946,12 @@ namespace sg {
 constexpr int kFusedRBW = 1, kFusedNW = 8;
 size_t fused_lds() { return (size_t)kNH * kRowPad * sizeof(__bf16); }
 int fused_rows_per_block() { return 32 * kFusedRBW * kFusedNW; }
-hipError_t prep_fused(const float *W1, const float *b1, const float *W5, const float *bh0, size_t bstride,
-                      __bf16 *a1h, __bf16 *a1l, __bf16 *a5h, __bf16 *a5l, float *bh, hipStream_t st) {
-    const int n = kNH * 16 + kNOUT * kNH + kNG * kNH;
-    hipLaunchKernelGGL(k_prep_fused, dim3(blocks(n)), dim3(256), 0, st, W1, b1, W5, bh0, bstride, a1h, a1l, a5h,
-                       a5l, bh);
+hipError_t prep_fused(const float *W1, const float *b1, const float *W5, const float *Wh0, const float *bh0,
+                      size_t bstride, __bf16 *a1h, __bf16 *a1l, _Float16 *wt, _Float16 *a5h, _Float16 *a5l,
+                      float *bh, hipStream_t st) {
+    const int n = kNH * 16 + kNOUT * kNH + kNG * kNH + kNG * kNH * kNH;
+    hipLaunchKernelGGL(k_prep_fused, dim3(blocks(n)), dim3(256), 0, st, W1, b1, W5, Wh0, bh0, bstride, a1h, a1l,
+                       wt, a5h, a5l, bh);
     return hipGetLastError();
 }
 hipError_t mlp_infer(const InferSet *sets, int nsets, int ntiles, const FusedWeights &fw, const TargetNorm &tn,
