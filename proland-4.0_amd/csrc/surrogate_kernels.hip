@@ -554,7 +554,8 @@ __global__ __launch_bounds__(256) void k_absmax(const float *__restrict__ a, siz
 // the activations in registers.
 //
 // The products are computed transposed, D = W^T H^T (output features x rows),
-// with v_mfma_f32_32x32x16_bf16: the A operand is a 32-feature x 16-k slice of
+// with v_mfma_f32_32x32x16_f16 (FP16 weights and activations, FP32
+// accumulation; layer 1 with the _bf16 form): the A operand is a 32-feature x 16-k slice of
 // the weight (W^T rows, staged in LDS), the B operand a 16-k x 32-row slice of
 // the activations, held by each lane for its own row (lane l: row l%32,
 // features 16kb + 8(l/32) .. +7 of every k block kb).  A D tile gives lane l
@@ -562,12 +563,12 @@ __global__ __launch_bounds__(256) void k_absmax(const float *__restrict__ a, siz
 // with lane l^32 turns it into the B layout of the next layer.  Each wave owns
 // 32*RBW rows through all five layers; a workgroup stages one 256x256 weight
 // (rows padded to 528 B, conflict-free 16-byte LDS reads) per layer.
-// Layers 1 and 5 use the same MFMA with FP32 weights split into bf16 hi + lo
-// parts (x W ~ xh Wh + xh Wl + xl Wh for layer 1, whose bias rides in input
-// column 14; H4 is bf16 already, so layer 5 needs Wh + Wl only): relative
-// error ~1e-5, well inside the bf16 rounding of the next activation.
+// Layers 1 and 5 split their FP32 weights into hi + lo parts, bf16 for layer 1
+// (x W ~ xh Wh + xh Wl + xl Wh, its bias riding in input column 14) and FP16
+// for layer 5 (its input is FP16 already, so Wh + Wl only): relative error
+// ~1e-5 and ~1e-6, inside the FP16 rounding of the activations.
 // ---------------------------------------------------------------------------
-constexpr int kRowPad = 264;   // bf16 per staged weight row (512 B + 16 B pad)
+constexpr int kRowPad = 264;   // 16-bit values per staged weight row (512 B + 16 B pad)
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
@@ -625,7 +626,7 @@ __device__ __forceinline__ void tile_to_b(const f32x16 &acc, const float *bias, 
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// Asynchronous copy of rows [r0, r0 + 128) of a [256][256] bf16 weight into
+// Asynchronous copy of rows [r0, r0 + 128) of a [256][256] FP16 weight into
 // the padded LDS image (global_load_lds_dwordx4: each active lane moves 16 B
 // straight into LDS; one row = 32 lanes).  Each of the NW waves moves 128/NW rows.
 template <int NW>
