@@ -1016,7 +1016,9 @@ __device__ __forceinline__ float sn_txs(const SetDesc &s, float nu, float mus, f
 // `sb` samples are blended per LDS round (dynamic LDS: sb * rows * RNU*RMU_S
 // float4).  A workgroup covers rows of RNU*RMU_S <= 256 texels with one texel
 // per thread, or one row of more with 2-4 texels per thread: the staged form
-// has a path for each.
+// has a path for each.  In the second, a sample whose 4 rows are those of the
+// sample before it in the same LDS round blends the loaded rows again (the
+// row starts are workgroup-uniform there, so the test is a scalar branch).
 // Three waves per SIMD (up to 168 VGPRs): at four (128 VGPRs) the staged form
 // spills 116 bytes per lane, whose traffic shows in the kernel's HBM counters,
 // and both builds run at the same speed (2.31 and 2.33 ms per launch at
@@ -1175,6 +1177,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 constexpr int NM = decltype(nmc)::value;
                 for (int s0 = 0; s0 < nc; s0 += sb) {
                     const int ns = min(sb, nc - s0);
+                    // the table rows of the last sample loaded in this round: a later
+                    // sample on the same 4 rows (same first and last row start, which fix
+                    // the two layers and the two mu rows) blends them again without
+                    // loading (kept within a round: carried over the lookups, t would
+                    // spill at three waves per SIMD)
+                    float3 t[NM][4];
+                    unsigned P0 = 0u, P3 = 0u;
                     for (int k = 0; k < ns; ++k) {
                         const float4 w = ent[0][s0 + k].w;
                         const int4 base = ent[0][s0 + k].base;
@@ -1185,15 +1194,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                         const unsigned R1 = (unsigned)__builtin_amdgcn_readfirstlane(base.y);
                         const unsigned R2 = (unsigned)__builtin_amdgcn_readfirstlane(base.z);
                         const unsigned R3 = (unsigned)__builtin_amdgcn_readfirstlane(base.w);
-                        float3 t[NM][4];
+                        if (k == 0 || R0 != P0 || R3 != P3) {
+                            P0 = R0;
+                            P3 = R3;
 #pragma unroll
-                        for (int j = 0; j < NM; ++j) {
-                            // past the row's end: a valid address, the blend is not stored
-                            const unsigned xo = (unsigned)min((int)threadIdx.x + j * (int)blockDim.x, per - 1) * jb;
-                            t[j][0] = ld3b(Jr, xo, R0);
-                            t[j][1] = ld3b(Jr, xo, R1);
-                            t[j][2] = ld3b(Jr, xo, R2);
-                            t[j][3] = ld3b(Jr, xo, R3);
+                            for (int j = 0; j < NM; ++j) {
+                                // past the row's end: a valid address, the blend is not stored
+                                const unsigned xo = (unsigned)min((int)threadIdx.x + j * (int)blockDim.x, per - 1) * jb;
+                                t[j][0] = ld3b(Jr, xo, R0);
+                                t[j][1] = ld3b(Jr, xo, R1);
+                                t[j][2] = ld3b(Jr, xo, R2);
+                                t[j][3] = ld3b(Jr, xo, R3);
+                            }
                         }
 #pragma unroll
                         for (int j = 0; j < NM; ++j) {
